@@ -1,0 +1,99 @@
+"""In-tree build of the native libraries (no JIT cache, no site-packages).
+
+Outputs under gpuattract_2.0_amd/lib/:
+  libattract_gen.so   seeded workload generator (host C++)
+  libattract_hip.so   HIP kernels for gfx950 + the C ABI of include/attract_hip.h
+  libattract_service.so / service_test
+                      C++ Server/Service drop-in (MI355EnergyService6D) over the C ABI
+"""
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+INC = os.path.join(ROOT, "include")
+LIB = os.path.join(PKG, "lib")
+CSRC = os.path.join(PKG, "csrc")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed-FP32 instructions are disabled for every HIP translation unit.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+    "-Wno-unknown-warning-option",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, cwd=ROOT, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout)
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    return r.stdout
+
+
+def build_gen():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libattract_gen.so")
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INC,
+          os.path.join(CSRC, "gen", "attract_gen.cpp"), "-o", out, "-pthread"])
+    return out
+
+
+def hip_sources():
+    d = os.path.join(CSRC, "hip")
+    return sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith((".hip", ".cpp")))
+
+
+def build_hip():
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libattract_hip.so")
+    srcs = hip_sources()
+    if not srcs:
+        return None
+    objs = []
+    for s in srcs:
+        o = os.path.join(LIB, os.path.basename(s) + ".o")
+        _run([HIPCC] + HIP_FLAGS + ["-I", INC, "-c", s, "-o", o])
+        objs.append(o)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", out] + objs)
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_service():
+    d = os.path.join(CSRC, "service")
+    srcs = sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cpp"))
+    if not srcs:
+        return None
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libattract_service.so")
+    lib_srcs = [s for s in srcs if not os.path.basename(s).startswith("test_")]
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INC, "-I", d]
+         + lib_srcs + ["-o", out, "-L", LIB, "-lattract_hip", "-Wl,-rpath,$ORIGIN", "-pthread"])
+    tests = [s for s in srcs if os.path.basename(s).startswith("test_")]
+    for t in tests:
+        exe = os.path.join(LIB, os.path.splitext(os.path.basename(t))[0])
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, "-I", d, t, "-o", exe,
+              "-L", LIB, "-lattract_service", "-lattract_hip", "-Wl,-rpath,$ORIGIN", "-pthread"])
+    return out
+
+
+def build_oracle():
+    """Test-only checkers (oracle/liboracle.so and, with the reference tree, oracle/_ref)."""
+    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "all"])
+
+
+def build_all():
+    build_gen()
+    build_hip()
+    build_service()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    build_all()

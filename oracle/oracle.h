@@ -1,0 +1,113 @@
+/*
+ * oracle.h — CPU restatement of gpuATTRACT's pose-scoring path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker for the HIP library and
+ * the CPU baseline of the benchmark; no product code links or calls it.
+ *
+ * Follows the reference's CPU service (service/CPUEnergyService6D.tpp:62-204,
+ * service/CPUEnergyService6DModes.tpp:121-327) and the model headers it calls,
+ * in FP32 with the reference's double promotions, plus an FP64 recompute of
+ * the same algorithm.  One deliberate deviation: sin/cos are evaluated as the
+ * correctly rounded float of the double-precision value, because the
+ * reference's std::sin/std::cos(float) results come from the C library
+ * (glibc's sinf/cosf are not correctly rounded and differ by CPU variant).
+ */
+#ifndef ATTRACT_ORACLE_H_
+#define ATTRACT_ORACLE_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Interpolation grid (model/IntrplGrid.h:34-209). data = ngrids*nz*ny*nx float4. */
+typedef struct {
+    unsigned nx, ny, nz, ngrids;
+    float ox, oy, oz, spacing;
+    const float* data;
+} orc_igrid;
+
+/* Neighbour-list grid (model/NLGrid.h:36-202). desc = {numEl, idx} per voxel. */
+typedef struct {
+    unsigned nx, ny, nz;
+    float ox, oy, oz, spacing, plateau;
+    const uint32_t* desc;
+    const uint32_t* list;
+} orc_nlgrid;
+
+/* Protein (model/Protein.h:38-167).  xyz SoA; modes [x:n*M][y:n*M][z:n*M]. */
+typedef struct {
+    unsigned n, M;
+    const float* xyz;
+    const unsigned* type;
+    const unsigned* mtype;
+    const float* charge;
+    const float* modes;
+    const float* modeForce;
+} orc_body;
+
+/* Pair table (model/ParamTable.h:18-101) + sim params (model/SimParam.h:39-54). */
+typedef struct {
+    unsigned T;
+    int shape; /* 8 or 12 */
+    int dielec; /* 0 constant, 1 variable */
+    float ffelec;
+    const float* rc;
+    const float* ac;
+    const int* ipon;
+    const float* rmin2;
+    const float* emin;
+} orc_params;
+
+/* Per-bead diagnostics of one pose (all arrays sized by the body). */
+typedef struct {
+    float* xyz;        /* transformed coordinates, SoA */
+    float* pot;        /* per-bead fx,fy,fz,E after grid + NL, SoA [4][n] */
+    int* cell;         /* per bead: inner idx xyz, outer idx xyz, NL idx xyz (9 ints), -1 if unused */
+    unsigned* nl_pairs; /* per bead: number of listed neighbours inside the plateau */
+    float* rec_pot;    /* modes only: receptor forces [3][n_rec] after grid rotate + NL */
+    float* rec_xyz;    /* modes only: receptor coordinates in the ligand frame */
+} orc_beads;
+
+/* FP32 plateau-ratio table (model/NLGrid.tpp:44-51); size int(1e4*P^2)+1. */
+unsigned orc_ratio_size(float plateau);
+void orc_ratio_lut(float plateau, float* out);
+
+void orc_euler2rotmat(float phi, float ssi, float rot, float R[9]);
+void orc_euler2torquemat(float phi, float ssi, float rot, float tm[27]);
+
+/* Rigid scoring: dof = n*6 {pos, ang}, res = n*7 {E, pos, ang} (Result_6D). */
+int orc_score_rigid(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                    const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                    const float* dof, size_t n, float* res, int nthreads);
+int orc_score_rigid_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                        const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                        const float* dof, size_t n, double* res, int nthreads);
+int orc_beads_rigid(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                    const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                    const float* dof, orc_beads* out);
+
+/* Modes scoring in the reference CPU semantics.  dof = n*26
+ * {pos, ang, modesRec[10], modesLig[10]} (DOF_6D_Modes), res = n*27
+ * {E, pos, ang, modesRec[10], modesLig[10]} (Result_6D_Modes). */
+int orc_score_modes(const orc_body* rec, const orc_body* lig, const orc_igrid* recInner,
+                    const orc_igrid* recOuter, const orc_nlgrid* recNL,
+                    const orc_igrid* ligInner, const orc_igrid* ligOuter, const orc_params* par,
+                    const float* dof, size_t n, float* res, int nthreads);
+int orc_score_modes_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* recInner,
+                        const orc_igrid* recOuter, const orc_nlgrid* recNL,
+                        const orc_igrid* ligInner, const orc_igrid* ligOuter,
+                        const orc_params* par, const float* dof, size_t n, double* res,
+                        int nthreads);
+int orc_beads_modes(const orc_body* rec, const orc_body* lig, const orc_igrid* recInner,
+                    const orc_igrid* recOuter, const orc_nlgrid* recNL,
+                    const orc_igrid* ligInner, const orc_igrid* ligOuter, const orc_params* par,
+                    const float* dof, orc_beads* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,301 @@
+"""ctypes bindings to the CPU oracle (liboracle.so) and the compiled reference
+harness (_ref/libattract_ref.so).
+
+TEST INFRASTRUCTURE ONLY — imported by tests/, the smoke check and the
+benchmark's CPU-baseline leg, never by product code.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+P = C.c_void_p
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class IGridC(C.Structure):
+    _fields_ = [("nx", C.c_uint), ("ny", C.c_uint), ("nz", C.c_uint), ("ngrids", C.c_uint),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("spacing", C.c_float),
+                ("data", P)]
+
+
+class NLGridC(C.Structure):
+    _fields_ = [("nx", C.c_uint), ("ny", C.c_uint), ("nz", C.c_uint),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("spacing", C.c_float),
+                ("plateau", C.c_float), ("desc", P), ("list", P), ("nlist", C.c_uint64)]
+
+
+class BodyC(C.Structure):
+    _fields_ = [("n", C.c_uint), ("M", C.c_uint), ("xyz", P), ("type", P), ("mtype", P),
+                ("charge", P), ("modes", P), ("modeForce", P)]
+
+
+class OrcParamsC(C.Structure):
+    _fields_ = [("T", C.c_uint), ("shape", C.c_int), ("dielec", C.c_int), ("ffelec", C.c_float),
+                ("rc", P), ("ac", P), ("ipon", P), ("rmin2", P), ("emin", P)]
+
+
+class RefTableC(C.Structure):
+    _fields_ = [("T", C.c_uint), ("shape", C.c_int), ("dielec", C.c_int),
+                ("epsilon", C.c_float), ("ffelec", C.c_float),
+                ("rc", P), ("ac", P), ("ipon", P), ("rmin2", P), ("emin", P)]
+
+
+class BeadsC(C.Structure):
+    _fields_ = [("xyz", P), ("pot", P), ("cell", P), ("nl_pairs", P), ("rec_pot", P),
+                ("rec_xyz", P)]
+
+
+def igrid_c(g):
+    ge = g.geom
+    return IGridC(ge.nx, ge.ny, ge.nz, g.ngrids, ge.ox, ge.oy, ge.oz, ge.spacing, _p(g.data))
+
+
+def nl_c(g):
+    ge = g.geom
+    return NLGridC(ge.nx, ge.ny, ge.nz, ge.ox, ge.oy, ge.oz, ge.spacing, g.plateau,
+                   _p(g.desc), _p(g.list), len(g.list))
+
+
+def body_c(b):
+    return BodyC(b.n, b.M, _p(b.xyz), _p(b.type), _p(b.mtype), _p(b.charge),
+                 _p(b.modes) if b.M else None, _p(b.mode_force) if b.M else None)
+
+
+def params_c(p):
+    return OrcParamsC(p.T, p.shape, p.dielec, p.ffelec, _p(p.rc), _p(p.ac), _p(p.ipon),
+                      _p(p.rmin2), _p(p.emin))
+
+
+def table_c(p):
+    return RefTableC(p.T, p.shape, p.dielec, p.epsilon, p.ffelec, _p(p.rc), _p(p.ac),
+                     _p(p.ipon), _p(p.rmin2), _p(p.emin))
+
+
+_orc = None
+_ref = None
+
+
+def oracle_lib():
+    global _orc
+    if _orc is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: build the oracle (make -C oracle)")
+        L = C.CDLL(path)
+        for name in ("orc_score_rigid", "orc_score_rigid_f64"):
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                          C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(OrcParamsC), P,
+                          C.c_size_t, P, C.c_int]
+        L.orc_beads_rigid.restype = C.c_int
+        L.orc_beads_rigid.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC),
+                                      C.POINTER(OrcParamsC), P, C.POINTER(BeadsC)]
+        for name in ("orc_score_modes", "orc_score_modes_f64"):
+            f = getattr(L, name)
+            f.restype = C.c_int
+            f.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                          C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
+                          C.POINTER(IGridC), C.POINTER(OrcParamsC), P, C.c_size_t, P, C.c_int]
+        L.orc_beads_modes.restype = C.c_int
+        L.orc_beads_modes.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(OrcParamsC), P,
+                                      C.POINTER(BeadsC)]
+        L.orc_ratio_size.restype = C.c_uint
+        L.orc_ratio_size.argtypes = [C.c_float]
+        L.orc_ratio_lut.restype = None
+        L.orc_ratio_lut.argtypes = [C.c_float, P]
+        L.orc_euler2rotmat.restype = None
+        L.orc_euler2rotmat.argtypes = [C.c_float, C.c_float, C.c_float, P]
+        L.orc_euler2torquemat.restype = None
+        L.orc_euler2torquemat.argtypes = [C.c_float, C.c_float, C.c_float, P]
+        _orc = L
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libattract_ref.so"))
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(os.path.join(HERE, "_ref", "libattract_ref.so"))
+        L.ref_score_rigid.restype = C.c_int
+        L.ref_score_rigid.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC),
+                                      C.POINTER(RefTableC), P, C.c_size_t, P]
+        L.ref_beads_rigid.restype = C.c_int
+        L.ref_beads_rigid.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC),
+                                      C.POINTER(RefTableC), P, P, P]
+        L.ref_score_modes.restype = C.c_int
+        L.ref_score_modes.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC),
+                                      C.POINTER(RefTableC), P, C.c_size_t, P, P, P]
+        L.ref_read_params.restype = C.c_int
+        L.ref_read_params.argtypes = [C.c_uint, C.c_int, P, P, P, P, P, P, P, P]
+        L.ref_ratio_lut.restype = C.c_uint
+        L.ref_ratio_lut.argtypes = [C.c_float, P, C.c_uint]
+        L.ref_euler2rotmat.restype = None
+        L.ref_euler2rotmat.argtypes = [C.c_float, C.c_float, C.c_float, P]
+        L.ref_euler2torquemat.restype = None
+        L.ref_euler2torquemat.argtypes = [C.c_float, C.c_float, C.c_float, P]
+        _ref = L
+    return _ref
+
+
+# ---------------------------------------------------------------- oracle API
+
+def score_rigid(wl, dofs=None, nthreads=1, f64=False):
+    L = oracle_lib()
+    d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
+    n = d.shape[0]
+    res = np.zeros((n, 7), np.float64 if f64 else np.float32)
+    f = L.orc_score_rigid_f64 if f64 else L.orc_score_rigid
+    f(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)), C.byref(igrid_c(wl.rec_inner)),
+      C.byref(igrid_c(wl.rec_outer)), C.byref(nl_c(wl.rec_nl)), C.byref(params_c(wl.params)),
+      _p(d), n, _p(res), nthreads)
+    return res
+
+
+def score_modes(wl, dofs=None, nthreads=1, f64=False):
+    L = oracle_lib()
+    d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
+    n = d.shape[0]
+    res = np.zeros((n, 27), np.float64 if f64 else np.float32)
+    f = L.orc_score_modes_f64 if f64 else L.orc_score_modes
+    f(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)), C.byref(igrid_c(wl.rec_inner)),
+      C.byref(igrid_c(wl.rec_outer)), C.byref(nl_c(wl.rec_nl)), C.byref(igrid_c(wl.lig_inner)),
+      C.byref(igrid_c(wl.lig_outer)), C.byref(params_c(wl.params)), _p(d), n, _p(res), nthreads)
+    return res
+
+
+def beads(wl, dof):
+    """Per-bead diagnostics of one pose: transformed coordinates, forces/energy,
+    cell indices (inner xyz, outer xyz, NL xyz; -1 unused) and NL pair counts."""
+    L = oracle_lib()
+    n, nr = wl.lig.n, wl.rec.n
+    out = dict(xyz=np.zeros(3 * n, np.float32), pot=np.zeros(4 * n, np.float32),
+               cell=np.full(9 * n, -1, np.int32), nl_pairs=np.zeros(n, np.uint32))
+    modes = wl.modes
+    if modes:
+        out["rec_pot"] = np.zeros(3 * nr, np.float32)
+        out["rec_xyz"] = np.zeros(3 * nr, np.float32)
+    b = BeadsC(_p(out["xyz"]), _p(out["pot"]), _p(out["cell"]), _p(out["nl_pairs"]),
+               _p(out.get("rec_pot")), _p(out.get("rec_xyz")))
+    d = np.ascontiguousarray(dof, np.float32)
+    if modes:
+        L.orc_beads_modes(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                          C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                          C.byref(nl_c(wl.rec_nl)), C.byref(igrid_c(wl.lig_inner)),
+                          C.byref(igrid_c(wl.lig_outer)), C.byref(params_c(wl.params)), _p(d),
+                          C.byref(b))
+    else:
+        L.orc_beads_rigid(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                          C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                          C.byref(nl_c(wl.rec_nl)), C.byref(params_c(wl.params)), _p(d),
+                          C.byref(b))
+    out["cell"] = out["cell"].reshape(n, 9)
+    return out
+
+
+def rotmat(phi, ssi, rot):
+    R = np.zeros(9, np.float32)
+    oracle_lib().orc_euler2rotmat(phi, ssi, rot, _p(R))
+    return R
+
+
+def torquemat(phi, ssi, rot):
+    t = np.zeros(27, np.float32)
+    oracle_lib().orc_euler2torquemat(phi, ssi, rot, _p(t))
+    return t
+
+
+def ratio_lut(plateau):
+    L = oracle_lib()
+    out = np.zeros(L.orc_ratio_size(plateau), np.float32)
+    L.orc_ratio_lut(plateau, _p(out))
+    return out
+
+
+# ------------------------------------------------------------- reference API
+
+def ref_score_rigid(wl, dofs=None):
+    L = ref_lib()
+    d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
+    res = np.zeros((d.shape[0], 7), np.float32)
+    rc = L.ref_score_rigid(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                           C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                           C.byref(nl_c(wl.rec_nl)), C.byref(table_c(wl.params)), _p(d),
+                           d.shape[0], _p(res))
+    assert rc == 0
+    return res
+
+
+def ref_beads_rigid(wl, dof):
+    L = ref_lib()
+    n = wl.lig.n
+    xyz = np.zeros(3 * n, np.float32)
+    pot = np.zeros(4 * n, np.float32)
+    L.ref_beads_rigid(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                      C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                      C.byref(nl_c(wl.rec_nl)), C.byref(table_c(wl.params)),
+                      _p(np.ascontiguousarray(dof, np.float32)), _p(xyz), _p(pot))
+    return xyz, pot
+
+
+def ref_score_modes(wl, dofs=None):
+    L = ref_lib()
+    d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
+    res = np.zeros((d.shape[0], 27), np.float32)
+    rec_pot = np.zeros(3 * wl.rec.n, np.float32)
+    lig_pot = np.zeros(4 * wl.lig.n, np.float32)
+    rc = L.ref_score_modes(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                           C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                           C.byref(nl_c(wl.rec_nl)), C.byref(igrid_c(wl.lig_inner)),
+                           C.byref(igrid_c(wl.lig_outer)), C.byref(nl_c(wl.lig_nl)),
+                           C.byref(table_c(wl.params)), _p(d), d.shape[0], _p(res),
+                           _p(rec_pot), _p(lig_pot))
+    assert rc == 0
+    return res, rec_pot, lig_pot
+
+
+def ref_read_params(p):
+    L = ref_lib()
+    T = p.T
+    out = dict(rc=np.zeros(T * T, np.float32), ac=np.zeros(T * T, np.float32),
+               ipon=np.zeros(T * T, np.int32), rmin2=np.zeros(T * T, np.float32),
+               emin=np.zeros(T * T, np.float32))
+    rc = L.ref_read_params(T, p.shape, _p(p.rbc), _p(p.abc), _p(p.ipon_raw), _p(out["rc"]),
+                           _p(out["ac"]), _p(out["ipon"]), _p(out["rmin2"]), _p(out["emin"]))
+    assert rc == 0
+    return out
+
+
+def ref_ratio_lut(plateau):
+    L = ref_lib()
+    size = L.ref_ratio_lut(plateau, None, 0)
+    out = np.zeros(size, np.float32)
+    L.ref_ratio_lut(plateau, _p(out), size)
+    return out
+
+
+def ref_rotmat(phi, ssi, rot):
+    R = np.zeros(9, np.float32)
+    ref_lib().ref_euler2rotmat(phi, ssi, rot, _p(R))
+    return R
+
+
+def ref_torquemat(phi, ssi, rot):
+    t = np.zeros(27, np.float32)
+    ref_lib().ref_euler2torquemat(phi, ssi, rot, _p(t))
+    return t

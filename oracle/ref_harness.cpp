@@ -1,0 +1,321 @@
+// ref_harness.cpp — TEST INFRASTRUCTURE ONLY.
+//
+// A driver, written for this project, that runs the reference's own CPU code
+// (compiled from its sources under the reference tree by oracle/Makefile into
+// oracle/_ref/libattract_ref.so) on in-memory inputs, so that the CPU
+// restatement in oracle.c can be checked against it.  Nothing here is shipped
+// or called by the product; tests/golden/make_golden.py uses it to write the
+// committed golden vectors.
+//
+//  rigid scoring  : the reference's CPUEnergyService6D<float> item processor
+//                   (service/CPUEnergyService6D.tpp:62-204) on a WorkItem
+//  modes scoring  : the model functions in the order of
+//                   service/CPUEnergyService6DModes.tpp:121-327 (the service
+//                   object itself sizes its buffers crosswise and cannot run
+//                   with n_rec != n_lig)
+//  param table    : fileIO/readFile.cpp:439-527 on a written attract.par
+//  plateau ratios : model/NLGrid.tpp:44-51
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <type_traits>
+#include <unistd.h>
+
+#include "CPUEnergyService6D.h"
+#include "DataManager.h"
+#include "GridUnion.h"
+#include "IntrplGrid.h"
+#include "NLGrid.h"
+#include "ParamTable.h"
+#include "Protein.h"
+#include "SimParam.h"
+#include "Types_6D.h"
+#include "Types_6D_Modes.h"
+#include "WorkItem.h"
+#include "interpolation.h"
+#include "matrixFunctions.h"
+#include "neighborlist_modes.h"
+#include "readFile.h"
+#include "reduction_modes.h"
+#include "transform_modes.h"
+
+using namespace as;
+
+namespace {
+
+struct GridIn {
+    unsigned nx, ny, nz, ngrids;
+    float ox, oy, oz, spacing;
+    const float* data;
+};
+
+struct NLIn {
+    unsigned nx, ny, nz;
+    float ox, oy, oz, spacing, plateau;
+    const uint32_t* desc;
+    const uint32_t* list;
+    uint64_t nlist;
+};
+
+struct BodyIn {
+    unsigned n, M;
+    const float* xyz;
+    const unsigned* type;
+    const unsigned* mtype;
+    const float* charge;
+    const float* modes;
+    const float* modeForce;
+};
+
+struct TableIn {
+    unsigned T;
+    int shape, dielec;
+    float epsilon, ffelec;
+    const float* rc;
+    const float* ac;
+    const int* ipon;
+    const float* rmin2;
+    const float* emin;
+};
+
+std::shared_ptr<Protein<float>> make_protein(const BodyIn& b) {
+    auto p = std::make_shared<Protein<float>>();
+    p->setNumAtoms(b.n);
+    std::copy(b.xyz, b.xyz + 3 * (size_t)b.n, p->getOrCreatePosPtr());
+    std::copy(b.type, b.type + b.n, p->getOrCreateTypePtr());
+    p->setNumMappedTypes(1);
+    std::copy(b.mtype, b.mtype + b.n, p->getOrCreateMappedPtr());
+    std::copy(b.charge, b.charge + b.n, p->getOrCreateChargePtr());
+    if (b.M > 0) {
+        p->setNumModes(b.M);
+        std::copy(b.modes, b.modes + 3 * (size_t)b.n * b.M, p->getOrCreateModePtr());
+        std::copy(b.modeForce, b.modeForce + b.M, p->getOrCreateModeForcePtr());
+    }
+    return p;
+}
+
+std::shared_ptr<IntrplGrid<float>> make_igrid(const GridIn& g) {
+    IntrplGrid<float>::Desc d;
+    std::memset(d.typemask, 0, sizeof(d.typemask));
+    d.numGrids = g.ngrids;
+    d.dimN.x = g.nx; d.dimN.y = g.ny; d.dimN.z = g.nz;
+    d.gridSpacing = g.spacing;
+    d.posMin.x = g.ox; d.posMin.y = g.oy; d.posMin.z = g.oz;
+    const size_t cnt = (size_t)g.ngrids * g.nx * g.ny * g.nz;
+    d.grid = new float4[cnt];
+    std::memcpy(d.grid, g.data, cnt * sizeof(float4));
+    return std::make_shared<IntrplGrid<float>>(d);
+}
+
+std::shared_ptr<NLGrid<float>> make_nlgrid(const NLIn& g) {
+    NLGrid<float>::Desc d;
+    d.dimN.x = g.nx; d.dimN.y = g.ny; d.dimN.z = g.nz;
+    d.gridSpacing = g.spacing;
+    d.dPlateau = g.plateau;
+    d.posMin.x = g.ox; d.posMin.y = g.oy; d.posMin.z = g.oz;
+    const size_t nv = (size_t)g.nx * g.ny * g.nz;
+    d.grid = new NeighbourDesc[nv];
+    for (size_t v = 0; v < nv; ++v) { d.grid[v].numEl = g.desc[2 * v]; d.grid[v].idx = g.desc[2 * v + 1]; }
+    d.numEl = (unsigned)g.nlist;
+    d.neighborArray = new unsigned[g.nlist > 0 ? g.nlist : 1];
+    std::copy(g.list, g.list + g.nlist, d.neighborArray);
+    return std::make_shared<NLGrid<float>>(d);
+}
+
+std::shared_ptr<ParamTable<float>> make_table(const TableIn& t) {
+    auto p = std::make_shared<ParamTable<float>>();
+    p->setNumTypes(t.T);
+    p->setPotShape(t.shape == 8 ? PotShape::_8_6 : PotShape::_12_6);
+    auto* tab = p->getOrCreateTable();
+    for (unsigned k = 0; k < t.T * t.T; ++k) {
+        tab[k].rc = t.rc[k];
+        tab[k].ac = t.ac[k];
+        tab[k].ipon = t.ipon[k];
+        tab[k].rmin2 = t.rmin2[k];
+        tab[k].emin = t.emin[k];
+    }
+    return p;
+}
+
+std::shared_ptr<SimParam<float>> make_sim(const TableIn& t) {
+    auto s = std::make_shared<SimParam<float>>();
+    s->dielec = t.dielec == 0 ? Dielec::constant : Dielec::variable;
+    s->epsilon = t.epsilon;
+    s->ffelec = t.ffelec;
+    return s;
+}
+
+}  // namespace
+
+// Overload resolution the reference's headers see for unqualified abs/sqrt on a
+// float inside namespace as (model/neighborlist.h:115, model/forcefield.h:104).
+namespace as {
+inline int ref_abs_probe() { return std::is_same<decltype(abs(1.5f)), float>::value ? 1 : 0; }
+inline int ref_sqrt_probe() { return std::is_same<decltype(sqrt(1.5f)), float>::value ? 1 : 0; }
+}  // namespace as
+
+extern "C" {
+
+int ref_abs_is_float() { return as::ref_abs_probe(); }
+int ref_sqrt_is_float() { return as::ref_sqrt_probe(); }
+
+void ref_euler2rotmat(float phi, float ssi, float rot, float* R) {
+    auto m = euler2rotmat<float>(phi, ssi, rot);
+    for (int i = 0; i < 9; ++i) R[i] = m.mat[i];
+}
+
+void ref_euler2torquemat(float phi, float ssi, float rot, float* out) {
+    auto t = euler2torquemat<float>(phi, ssi, rot);
+    for (int k = 0; k < 3; ++k)
+        for (int j = 0; j < 3; ++j)
+            for (int l = 0; l < 3; ++l) out[9 * k + 3 * j + l] = t.mat[k][j][l];
+}
+
+unsigned ref_ratio_lut(float plateau, float* out, unsigned cap) {
+    NLIn g{2, 2, 2, 0.f, 0.f, 0.f, 1.f, plateau, nullptr, nullptr, 0};
+    uint32_t desc[16] = {0};
+    g.desc = desc;
+    auto nl = make_nlgrid(g);
+    const unsigned size = (unsigned)int(10000 * nl->dPlateau2()) + 1;
+    for (unsigned n = 0; n < size && n < cap; ++n) out[n] = nl->_ratio[n];
+    return size;
+}
+
+// readParamTableFromFile on a temporary attract.par holding (rbc, abc, ipon).
+int ref_read_params(unsigned T, int shape, const float* rbc, const float* abc, const int* ipon,
+                    float* rc, float* ac, int* ipon_out, float* rmin2, float* emin) {
+    char path[] = "/tmp/attract_par_XXXXXX";
+    int fd = mkstemp(path);
+    if (fd < 0) return -1;
+    FILE* f = fdopen(fd, "w");
+    std::fprintf(f, "%d %u 0 0\n", shape, T);
+    for (unsigned k = 0; k < T * T; ++k) std::fprintf(f, "%.9g%c", rbc[k], (k % T == T - 1) ? '\n' : ' ');
+    for (unsigned k = 0; k < T * T; ++k) std::fprintf(f, "%.9g%c", abc[k], (k % T == T - 1) ? '\n' : ' ');
+    for (unsigned k = 0; k < T * T; ++k) std::fprintf(f, "%d%c", ipon[k], (k % T == T - 1) ? '\n' : ' ');
+    std::fclose(f);
+    auto tab = std::make_shared<ParamTable<float>>();
+    readParamTableFromFile<float>(tab, path);
+    unlink(path);
+    for (unsigned k = 0; k < T * T; ++k) {
+        const auto& p = tab->table()[k];
+        rc[k] = p.rc; ac[k] = p.ac; ipon_out[k] = p.ipon; rmin2[k] = p.rmin2; emin[k] = p.emin;
+    }
+    return 0;
+}
+
+// Rigid scoring through the reference's CPU item processor.
+int ref_score_rigid(const BodyIn* rec, const BodyIn* lig, const GridIn* inner, const GridIn* outer,
+                    const NLIn* nl, const TableIn* tab, const float* dof, size_t n, float* res) {
+    auto dm = std::make_shared<DataManager>();
+    auto gu = std::make_shared<GridUnion<float>>(make_igrid(*inner), make_igrid(*outer), make_nlgrid(*nl));
+    Common common;
+    common.gridId = dm->add(gu);
+    common.ligId = dm->add(make_protein(*lig));
+    common.recId = dm->add(make_protein(*rec));
+    common.tableId = dm->add(make_table(*tab));
+    common.paramsId = dm->add(make_sim(*tab));
+    CPUEnergyService6D<float> service(dm);
+    auto proc = service.createItemProcessor();
+    std::vector<DOF_6D<float>> in(n);
+    std::vector<Result_6D<float>> out(n);
+    for (size_t i = 0; i < n; ++i) {
+        in[i].pos = Vec3<float>(dof[6 * i], dof[6 * i + 1], dof[6 * i + 2]);
+        in[i].ang = Vec3<float>(dof[6 * i + 3], dof[6 * i + 4], dof[6 * i + 5]);
+    }
+    for (size_t lo = 0; lo < n; lo += 100) {
+        unsigned sz = (unsigned)std::min<size_t>(100, n - lo);
+        WorkItem<DOF_6D<float>, Common, Result_6D<float>> item(in.data() + lo, sz, &common, out.data() + lo);
+        proc(&item);
+        if (!item.isProcessed()) return -1;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        res[7 * i] = out[i].E;
+        res[7 * i + 1] = out[i].pos.x; res[7 * i + 2] = out[i].pos.y; res[7 * i + 3] = out[i].pos.z;
+        res[7 * i + 4] = out[i].ang.x; res[7 * i + 5] = out[i].ang.y; res[7 * i + 6] = out[i].ang.z;
+    }
+    return 0;
+}
+
+// Per-bead forces of one rigid pose: rotate_translate -> potForce -> NLPotForce.
+int ref_beads_rigid(const BodyIn* rec, const BodyIn* lig, const GridIn* inner, const GridIn* outer,
+                    const NLIn* nl, const TableIn* tab, const float* dof, float* xyz, float* pot) {
+    auto pr = make_protein(*rec);
+    auto pl = make_protein(*lig);
+    auto gi = make_igrid(*inner);
+    auto go = make_igrid(*outer);
+    auto gn = make_nlgrid(*nl);
+    auto pt = make_table(*tab);
+    auto sp = make_sim(*tab);
+    const unsigned N = pl->numAtoms();
+    Vec3<float> pos(dof[0], dof[1], dof[2]), ang(dof[3], dof[4], dof[5]);
+    rotate_translate(pl->xPos(), pl->yPos(), pl->zPos(), pos, ang, N, xyz, xyz + N, xyz + 2 * N);
+    potForce(gi.get(), go.get(), pl.get(), xyz, xyz + N, xyz + 2 * N, pot, pot + N, pot + 2 * N, pot + 3 * N);
+    NLPotForce(gn.get(), pr.get(), pl.get(), sp.get(), pt.get(), xyz, xyz + N, xyz + 2 * N, pot,
+               pot + N, pot + 2 * N, pot + 3 * N);
+    return 0;
+}
+
+// Modes scoring: model functions in the order of CPUEnergyService6DModes.tpp:121-327,
+// buffers sized by their own protein.  res = 27 floats per pose.
+int ref_score_modes(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner,
+                    const GridIn* recOuter, const NLIn* recNL, const GridIn* ligInner,
+                    const GridIn* ligOuter, const NLIn* ligNL, const TableIn* tab, const float* dof,
+                    size_t n, float* res, float* recPotOut, float* ligPotOut) {
+    auto pr = make_protein(*rec);
+    auto pl = make_protein(*lig);
+    GridUnion<float> gridRec(make_igrid(*recInner), make_igrid(*recOuter), make_nlgrid(*recNL));
+    GridUnion<float> gridLig(make_igrid(*ligInner), make_igrid(*ligOuter), make_nlgrid(*ligNL));
+    auto pt = make_table(*tab);
+    auto sp = make_sim(*tab);
+    const unsigned NR = pr->numAtoms(), NL = pl->numAtoms();
+    std::vector<float> trafoRec(3 * NR), defoRec(3 * NR), potRec(4 * NR), trafoLig(3 * NL), potLig(4 * NL);
+    for (size_t i = 0; i < n; ++i) {
+        DOF_6D_Modes<float> d;
+        const float* s = dof + 26 * i;
+        d._6D.pos = Vec3<float>(s[0], s[1], s[2]);
+        d._6D.ang = Vec3<float>(s[3], s[4], s[5]);
+        for (int m = 0; m < MODES_MAX_NUMBER; ++m) { d.modesRec[m] = s[6 + m]; d.modesLig[m] = s[16 + m]; }
+        DOF_6D_Modes<float> inv = invertDOF(d);
+        rotate_translate_deform(pr->xPos(), pr->yPos(), pr->zPos(), inv._6D.pos, inv._6D.ang, NR,
+                                pr->numModes(), inv.modesRec, pr->xModes(), pr->yModes(), pr->zModes(),
+                                &defoRec[0], &defoRec[NR], &defoRec[2 * NR], &trafoRec[0],
+                                &trafoRec[NR], &trafoRec[2 * NR]);
+        rotate_translate(pl->xPos(), pl->yPos(), pl->zPos(), d._6D.pos, d._6D.ang, NL, pl->numModes(),
+                         d.modesLig, pl->xModes(), pl->yModes(), pl->zModes(), &trafoLig[0],
+                         &trafoLig[NL], &trafoLig[2 * NL]);
+        potForce(gridLig.inner.get(), gridLig.outer.get(), pr.get(), &trafoRec[0], &trafoRec[NR],
+                 &trafoRec[2 * NR], &potRec[0], &potRec[NR], &potRec[2 * NR], &potRec[3 * NR]);
+        rotate_forces(inv._6D.ang.inv(), NR, &potRec[0], &potRec[NR], &potRec[2 * NR]);
+        potForce(gridRec.inner.get(), gridRec.outer.get(), pl.get(), &trafoLig[0], &trafoLig[NL],
+                 &trafoLig[2 * NL], &potLig[0], &potLig[NL], &potLig[2 * NL], &potLig[3 * NL]);
+        NLPotForce(gridRec.NL.get(), pr.get(), pl.get(), sp.get(), pt.get(), &defoRec[0], &defoRec[NR],
+                   &defoRec[2 * NR], &trafoLig[0], &trafoLig[NL], &trafoLig[2 * NL], &potLig[0],
+                   &potLig[NL], &potLig[2 * NL], &potLig[3 * NL], &potRec[0], &potRec[NR], &potRec[2 * NR]);
+        PotForce_Modes<float> red = reducePotForce<float, PotForce_Modes<float>>(
+            &potLig[0], &potLig[NL], &potLig[2 * NL], &potLig[3 * NL], NL);
+        reduceModeForce(d._6D.ang, &potLig[0], &potLig[NL], &potLig[2 * NL], pl->xModes(), pl->yModes(),
+                        pl->zModes(), NL, pl->numModes(), red.modesLig);
+        correctModeForce(pl->modeForce(), pl->numModes(), red.modesLig);
+        reduceModeForce(&potRec[0], &potRec[NR], &potRec[2 * NR], pr->xModes(), pr->yModes(),
+                        pr->zModes(), NR, pr->numModes(), red.modesRec);
+        correctModeForce(pr->modeForce(), pr->numModes(), red.modesRec);
+        Vec3<float> a = reduceTorque(pl->xPos(), pl->yPos(), pl->zPos(), &potLig[0], &potLig[NL],
+                                     &potLig[2 * NL], NL, d._6D.ang);
+        float* o = res + 27 * i;
+        o[0] = red.E;
+        o[1] = red.pos.x; o[2] = red.pos.y; o[3] = red.pos.z;
+        o[4] = a.x; o[5] = a.y; o[6] = a.z;
+        for (int m = 0; m < MODES_MAX_NUMBER; ++m) {
+            o[7 + m] = m < (int)pr->numModes() ? red.modesRec[m] : 0.f;
+            o[17 + m] = m < (int)pl->numModes() ? red.modesLig[m] : 0.f;
+        }
+        if (recPotOut && i == 0) std::copy(potRec.begin(), potRec.begin() + 3 * NR, recPotOut);
+        if (ligPotOut && i == 0) std::copy(potLig.begin(), potLig.end(), ligPotOut);
+    }
+    return 0;
+}
+
+}  // extern "C"
