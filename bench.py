@@ -1,0 +1,180 @@
+"""Benchmark: poses/s of energy + DOF gradient for flexible scoring on one MI355X.
+
+Workload (default): the flexible-scoring configuration — a 3000-bead receptor
+and an 800-bead ligand (seeded synthetic bodies, 12 bead types, 20% charged),
+5 + 5 orthonormal modes, receptor and ligand grids at 0.875 A (2.6 A outer),
+neighbour list with a 10 A plateau, 100,000 poses.  One step scores all poses
+with DOFs already resident in HBM.
+
+Prints one JSON line with the metric, a roofline object for the dominant
+kernel (algorithmic bytes per launch / launch time from HIP events on the
+launch stream) and the CPU baseline (the oracle restatement on the host cores,
+bounded sample).
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "gpuattract_2.0_amd")
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def _package():
+    if "gpuattract_amd" in sys.modules:
+        return sys.modules["gpuattract_amd"]
+    spec = importlib.util.spec_from_file_location(
+        "gpuattract_amd", os.path.join(PKG_DIR, "__init__.py"),
+        submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["gpuattract_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make(args, W):
+    if args.workload == "modes":
+        return W.make_workload("flexible scoring 3000/800, 5+5 modes", 20261203, 3000, 800,
+                               args.poses, 5, 5)
+    return W.make_workload("rigid scoring 1000/300", 20261202, 1000, 300, args.poses)
+
+
+def cpu_baseline(wl, args):
+    """Oracle restatement on the host cores over a bounded sample of poses."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    n = min(args.cpu_poses, wl.dofs.shape[0])
+    d = wl.dofs[:n]
+    f = O.score_modes if wl.modes else O.score_rigid
+    f(wl, d[:min(n, 4 * threads)], nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    res = f(wl, d, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="poses/s", cores=threads, kind="port",
+                sample=f"{n} poses of the same workload, {threads} threads, "
+                       f"items of 100 poses, {dt:.1f} s"), res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["modes", "rigid"], default="modes")
+    ap.add_argument("--poses", type=int, default=100000)
+    ap.add_argument("--cpu-poses", type=int, default=20000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-poses", type=int, default=5000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    _package()
+    from gpuattract_amd import hip, traffic, workload as W
+
+    t0 = time.perf_counter()
+    wl = make(args, W)
+    log(f"workload built in {time.perf_counter() - t0:.1f} s: {wl.name}, {wl.dofs.shape[0]} poses")
+    ctx = hip.Context(0)
+    hs = ctx.upload_workload(wl)
+    dev = torch.device("cuda:0")
+    n = wl.dofs.shape[0]
+    out_w = 27 if wl.modes else 7
+    d_in = torch.from_numpy(np.ascontiguousarray(wl.dofs)).to(dev)
+    d_out = torch.zeros((n, out_w), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    score = ctx.score_modes_dev if wl.modes else ctx.score_rigid_dev
+
+    def step():
+        score(hs, d_in.data_ptr(), n, d_out.data_ptr(), sh)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"warm-up done ({args.warmup} steps)")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        step()
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    avg_launch = float(np.mean(launch_ms)) / 1e3
+    value = n * args.steps / wall
+    log(f"timed {args.steps} steps: {wall:.3f} s, {value:.4g} poses/s, launch avg {avg_launch*1e3:.3f} ms")
+
+    bpp, bpp_sd = traffic.bytes_per_pose(wl)
+    achieved = bpp * n / avg_launch
+    roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / HBM_PEAK, traffic=None,
+                    bytes_per_pose=bpp, kernel="k_modes" if wl.modes else "k_rigid")
+    prof = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(prof):
+        try:
+            roofline["traffic"] = json.load(open(prof)).get("hbm_bytes_per_launch")
+        except Exception:
+            pass
+
+    # The plain per-item path: host buffers, 100 poses per call (the reference's
+    # item size, server/Server.h:39), as the drop-in service issues them.
+    m = min(args.plain_poses, n)
+    host_score = ctx.score_modes if wl.modes else ctx.score_rigid
+    host_score(hs, wl.dofs[:100])
+    t0 = time.perf_counter()
+    for a in range(0, m, 100):
+        host_score(hs, wl.dofs[a:a + 100])
+    plain = m / (time.perf_counter() - t0)
+    log(f"plain item-100 host path: {plain:.4g} poses/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu, ref = cpu_baseline(wl, args)
+        gpu = d_out[:ref.shape[0]].cpu().numpy()
+        rel = np.abs(gpu - ref) / np.maximum(np.abs(ref), 1e-30)
+        cpu["gpu_vs_cpu_median_rel_err"] = float(np.median(rel[:, 0]))
+        log(f"cpu baseline: {cpu['value']:.4g} poses/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "poses/sec (energy+DOF gradient) on 1 MI355X, batched+adjoint vs plain, at 1e-5 rel err",
+        "value": value,
+        "unit": "poses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded generator: bodies, pair table, grids, poses, modes)",
+        "config": {
+            "workload": wl.name,
+            "n_rec": wl.rec.n, "n_lig": wl.lig.n, "modes_rec": wl.rec.M, "modes_lig": wl.lig.M,
+            "poses": n, "path": "fused forward kernel, device-resident DOFs/results",
+            "plain_item100_poses_per_s": plain,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,365 @@
+// Device-side data descriptors and the per-bead arithmetic of the scoring path.
+//
+// Every function reproduces the reference CPU path's FP32 operation order
+// (no contraction: the library is built with -ffp-contract=off) so that
+// per-bead values match the CPU restatement bit for bit; only the per-pose
+// sums use a different (wave-tree) order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "attract_hip.h"
+
+namespace att {
+
+constexpr int kWave = 64;
+constexpr unsigned kMaxTypesLDS = 32;  // pair table staged in LDS up to 32x32 types
+
+// Interpolation grid (model/IntrplGrid.h:34-209) with the constants its
+// constructor derives (model/IntrplGrid.tpp:30-41).
+struct DGrid {
+    const float4* data;
+    unsigned nx, ny, nz, nxy, nvox, ngrids;
+    float lox, loy, loz, hix, hiy, hiz;
+    float dVox, dVox_inv, vol_inv;
+};
+
+// Neighbour-list grid (model/NLGrid.h:36-202, model/NLGrid.tpp:30-52).
+struct DNL {
+    const uint2* desc;
+    const uint32_t* list;
+    const float* ratio;
+    unsigned nx, ny, nz;
+    float lox, loy, loz, hix, hiy, hiz;
+    float dVox_inv, P2, P2_inv;
+};
+
+// Protein (model/Protein.h:38-167): pos = {x, y, z, charge}.
+struct DProt {
+    const float4* pos;
+    const unsigned* type;
+    const unsigned* mtype;
+    const float* modes;   // [c][m][n]: coalesced over beads
+    const float* modesT;  // [n][m][c]: one contiguous run per bead (gathers)
+    const float* modeForce;
+    unsigned n, M;
+};
+
+struct DPar {
+    const att_pair_params* tab;  // T*T, index T*(recType-1)+(ligType-1)
+    unsigned T;
+    int shape;   // 8 or 12
+    int dielec;  // 0 constant, 1 variable
+    float ffelec;
+};
+
+struct V3 { float x, y, z; };
+
+// RotMat::operator*(Vec3) (commons/RotMat.h:122-129).
+__device__ __forceinline__ V3 matvec(const float* m, float x, float y, float z) {
+    V3 o;
+    o.x = m[0] * x + m[1] * y + m[2] * z;
+    o.y = m[3] * x + m[4] * y + m[5] * z;
+    o.z = m[6] * x + m[7] * y + m[8] * z;
+    return o;
+}
+
+// euler2rotmat (model/matrixFunctions.h:38-77) from the six trig values
+// {cSSI, cPHI, sSSI, sPHI, cROT, sROT}.
+__device__ __forceinline__ void rotmat_from_trig(float cSSI, float cPHI, float sSSI, float sPHI,
+                                                 float cROT, float sROT, float* m) {
+    const float cSSI_cPHI = cSSI * cPHI, cSSI_sPHI = cSSI * sPHI;
+    const float sSSI_cPHI = sSSI * cPHI, sSSI_sPHI = sSSI * sPHI;
+    m[0] = cROT * cSSI_cPHI + sROT * sPHI;
+    m[1] = sROT * cSSI_cPHI - cROT * sPHI;
+    m[2] = sSSI_cPHI;
+    m[3] = cROT * cSSI_sPHI - sROT * cPHI;
+    m[4] = sROT * cSSI_sPHI + cROT * cPHI;
+    m[5] = sSSI_sPHI;
+    m[6] = -cROT * sSSI;
+    m[7] = -sROT * sSSI;
+    m[8] = cSSI;
+}
+
+// euler2torquemat (model/matrixFunctions.h:79-127) contracted with the torque
+// tensor T[k][l] as TorqueMat::rotateReduce does (commons/TorqueMat.h:27-37).
+__device__ __forceinline__ void torque_reduce(float cSSI, float cPHI, float sSSI, float sPHI,
+                                              float cROT, float sROT, const float* T, float* out) {
+    const float cSSI_cPHI = cSSI * cPHI, cSSI_sPHI = cSSI * sPHI;
+    const float sSSI_cPHI = sSSI * cPHI, sSSI_sPHI = sSSI * sPHI;
+    float tm[27];
+    tm[0] = -cROT * cSSI_sPHI + sROT * cPHI;
+    tm[1] = -sROT * cSSI_sPHI - cROT * cPHI;
+    tm[2] = -sSSI_sPHI;
+    tm[9] = cROT * cSSI_cPHI + sROT * sPHI;
+    tm[10] = sROT * cSSI_cPHI - cROT * sPHI;
+    tm[11] = sSSI_cPHI;
+    tm[18] = 0.f; tm[19] = 0.f; tm[20] = 0.f;
+    tm[3] = -cROT * sSSI_cPHI;
+    tm[4] = -sROT * sSSI_cPHI;
+    tm[5] = cSSI_cPHI;
+    tm[12] = -cROT * sSSI_sPHI;
+    tm[13] = -sROT * sSSI_sPHI;
+    tm[14] = cSSI_sPHI;
+    tm[21] = -cROT * cSSI;
+    tm[22] = -sROT * cSSI;
+    tm[23] = -sSSI;
+    tm[6] = -sROT * cSSI_cPHI + cROT * sPHI;
+    tm[7] = cROT * cSSI_cPHI + sROT * sPHI;
+    tm[8] = 0.f;
+    tm[15] = -sROT * cSSI_sPHI - cROT * cPHI;
+    tm[16] = cROT * cSSI_sPHI - sROT * cPHI;
+    tm[17] = 0.f;
+    tm[24] = sROT * sSSI;
+    tm[25] = -cROT * sSSI;
+    tm[26] = 0.f;
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+            r0 += tm[9 * k + 0 + l] * T[3 * k + l];
+            r1 += tm[9 * k + 3 + l] * T[3 * k + l];
+            r2 += tm[9 * k + 6 + l] * T[3 * k + l];
+        }
+    out[0] = r0; out[1] = r1; out[2] = r2;
+}
+
+// The six trig values of a pose, evaluated once per wave: lanes 0..2 take
+// sincos of (ssi, phi, rot) in double and round to float (the correctly
+// rounded single-precision sin/cos), then every lane reads them back.
+__device__ __forceinline__ void pose_trig(float phi, float ssi, float rot, int lane, float* t) {
+    float c = 0.f, s = 0.f;
+    if (lane < 3) {
+        const double a = lane == 0 ? (double)ssi : lane == 1 ? (double)phi : (double)rot;
+        double sd, cd;
+        sincos(a, &sd, &cd);
+        c = (float)cd;
+        s = (float)sd;
+    }
+    t[0] = __shfl(c, 0);  // cSSI
+    t[1] = __shfl(c, 1);  // cPHI
+    t[2] = __shfl(s, 0);  // sSSI
+    t[3] = __shfl(s, 1);  // sPHI
+    t[4] = __shfl(c, 2);  // cROT
+    t[5] = __shfl(s, 2);  // sROT
+}
+
+// IntrplGrid::outOfBounds_byPos (model/IntrplGrid.h:113-117).
+// Written as "not inside" so that a NaN coordinate counts as outside (the
+// reference's form would index the grid with it); identical for finite input.
+__device__ __forceinline__ bool grid_out(const DGrid& g, float x, float y, float z) {
+    return !((x >= g.lox && x <= g.hix) && (y >= g.loy && y <= g.hiy) && (z >= g.loz && z <= g.hiz));
+}
+
+__device__ __forceinline__ unsigned grid_index(float p, float lo, float inv, unsigned n) {
+    const unsigned i = (unsigned)floorf((p - lo) * inv);
+    return i < n - 2 ? i : n - 2;
+}
+
+// getVoxelByIndex + trilinearInterpolation (model/IntrplGrid.h:130-163,
+// model/trilinIntrpl.h:28-65): eight float4 corners, weights and summation in
+// the reference's order, then scaled by 1/voxelVolume.
+__device__ __forceinline__ float4 trilin(const DGrid& g, float px, float py, float pz, unsigned ix,
+                                         unsigned iy, unsigned iz, unsigned type) {
+    const float mnx = (float)(int)ix * g.dVox + g.lox;
+    const float mny = (float)(int)iy * g.dVox + g.loy;
+    const float mnz = (float)(int)iz * g.dVox + g.loz;
+    const float ax = px - mnx, ay = py - mny, az = pz - mnz;
+    const float bx = (mnx + g.dVox) - px, by = (mny + g.dVox) - py, bz = (mnz + g.dVox) - pz;
+    const float tmpMax_xy = bx * by;
+    const float tmpMin_xy = ax * ay;
+    const float w0 = tmpMax_xy * bz;
+    const float w1 = (ax * by) * bz;
+    const float w2 = (bx * ay) * bz;
+    const float w3 = tmpMax_xy * az;
+    const float w4 = (ax * by) * az;
+    const float w5 = (bx * ay) * az;
+    const float w6 = tmpMin_xy * bz;
+    const float w7 = tmpMin_xy * az;
+    const unsigned base = type * g.nvox + g.nx * (iz * g.ny + iy) + ix;
+    const float4* d = g.data + base;
+    const float4 c0 = d[0], c1 = d[1], c2 = d[g.nx], c6 = d[g.nx + 1];
+    const float4 c3 = d[g.nxy], c4 = d[g.nxy + 1], c5 = d[g.nxy + g.nx], c7 = d[g.nxy + g.nx + 1];
+    float4 V;
+    V.x = c0.x * w0; V.y = c0.y * w0; V.z = c0.z * w0; V.w = c0.w * w0;
+    V.x = V.x + c1.x * w1; V.y = V.y + c1.y * w1; V.z = V.z + c1.z * w1; V.w = V.w + c1.w * w1;
+    V.x = V.x + c2.x * w2; V.y = V.y + c2.y * w2; V.z = V.z + c2.z * w2; V.w = V.w + c2.w * w2;
+    V.x = V.x + c3.x * w3; V.y = V.y + c3.y * w3; V.z = V.z + c3.z * w3; V.w = V.w + c3.w * w3;
+    V.x = V.x + c4.x * w4; V.y = V.y + c4.y * w4; V.z = V.z + c4.z * w4; V.w = V.w + c4.w * w4;
+    V.x = V.x + c5.x * w5; V.y = V.y + c5.y * w5; V.z = V.z + c5.z * w5; V.w = V.w + c5.w * w5;
+    V.x = V.x + c6.x * w6; V.y = V.y + c6.y * w6; V.z = V.z + c6.z * w6; V.w = V.w + c6.w * w6;
+    V.x = V.x + c7.x * w7; V.y = V.y + c7.y * w7; V.z = V.z + c7.z * w7; V.w = V.w + c7.w * w7;
+    V.x = V.x * g.vol_inv; V.y = V.y * g.vol_inv; V.z = V.z * g.vol_inv; V.w = V.w * g.vol_inv;
+    return V;
+}
+
+// interpolate (model/interpolation.h:26-53).
+__device__ __forceinline__ float4 interp(const DGrid& g, float px, float py, float pz,
+                                         unsigned type, float charge, int* cell) {
+    const unsigned ix = grid_index(px, g.lox, g.dVox_inv, g.nx);
+    const unsigned iy = grid_index(py, g.loy, g.dVox_inv, g.ny);
+    const unsigned iz = grid_index(pz, g.loz, g.dVox_inv, g.nz);
+    if (cell) { cell[0] = (int)ix; cell[1] = (int)iy; cell[2] = (int)iz; }
+    float4 pot = trilin(g, px, py, pz, ix, iy, iz, type);
+    if (fabsf(charge) > 0.001f) {
+        float4 el = trilin(g, px, py, pz, ix, iy, iz, 0);
+        el.x = el.x * charge; el.y = el.y * charge; el.z = el.z * charge; el.w = el.w * charge;
+        pot.x = pot.x + el.x; pot.y = pot.y + el.y; pot.z = pot.z + el.z; pot.w = pot.w + el.w;
+    }
+    return pot;
+}
+
+// potForce for one bead (model/interpolation.h:56-102): inner grid, else outer
+// grid, else zero; mapped type 0 contributes nothing.
+__device__ __forceinline__ float4 pot_bead(const DGrid& inner, const DGrid& outer, float px,
+                                           float py, float pz, unsigned mtype, float charge,
+                                           int* cell) {
+    float4 pot = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (mtype == 0) return pot;
+    if (grid_out(inner, px, py, pz)) {
+        if (!grid_out(outer, px, py, pz)) pot = interp(outer, px, py, pz, mtype, charge, cell ? cell + 3 : nullptr);
+    } else {
+        pot = interp(inner, px, py, pz, mtype, charge, cell);
+    }
+    return pot;
+}
+
+// LJPotForce (model/forcefield.h:28-83) with swi = 1: 6.0*vlj and the force
+// factor are evaluated in double, as the reference's FP32 build does.
+__device__ __forceinline__ void lj(float dr2, float dr2_inv, float dx, float dy, float dz,
+                                   const att_pair_params& p, int shape, float& fx, float& fy,
+                                   float& fz, float& e) {
+    const float ivor = (float)p.ipon;
+    const float rr23 = dr2_inv * dr2_inv * dr2_inv;
+    const float rrd = shape == 8 ? dr2_inv : rr23;
+    const float shapedelta = shape == 8 ? 2.f : 6.f;
+    const float rep = p.rc * rrd;
+    const float vlj = (rep - p.ac) * rr23;
+    const double s = 6.0 * (double)vlj + (double)(shapedelta * (rep * rr23));
+    float fb;
+    if (dr2 < p.rmin2) {
+        e = vlj + (ivor - 1.f) * p.emin;
+        fb = (float)s;
+    } else {
+        e = ivor * vlj;
+        fb = (float)((double)ivor * s);
+    }
+    fx = fb * dx; fy = fb * dy; fz = fb * dz;
+}
+
+// ChargePotForce (model/forcefield.h:85-146) with swi = 1.
+__device__ __forceinline__ void coulomb(float dr2_inv, float dx, float dy, float dz, float q,
+                                        int dielec, float& fx, float& fy, float& fz, float& e) {
+    float dd;
+    if (dielec == 0) dd = (float)((double)sqrtf(dr2_inv) - 1.0 / 50.0);
+    else dd = (float)((double)dr2_inv - 1.0 / (50.0 * 50.0));
+    if (dd < 0.f) dd = 0.f;
+    e = q * dd;
+    if (dd <= 0.f) { fx = 0.f; fy = 0.f; fz = 0.f; return; }
+    if (dielec == 0) {
+        const float et2 = q * sqrtf(dr2_inv);
+        fx = et2 * dx; fy = et2 * dy; fz = et2 * dz;
+    } else {
+        const float et2 = q * dr2_inv;
+        fx = (2.f * et2) * dx; fy = (2.f * et2) * dy; fz = (2.f * et2) * dz;
+    }
+}
+
+// Neighbour-list term of one ligand bead (model/neighborlist.h:47-158 and the
+// receptor reaction of model/neighborlist_modes.h:68-194).  RecPos(j) returns
+// the receptor position used for the distance; Reaction(j, fx, fy, fz) receives
+// the reaction force of pair j (no-op for rigid scoring).  Returns false when
+// the bead is outside the neighbour-list grid.
+template <class RecPos, class Reaction>
+__device__ __forceinline__ bool nl_bead(const DNL& nl, const DProt& rec, const DPar& par,
+                                        const att_pair_params* tab, RecPos recPos,
+                                        Reaction reaction, float px, float py, float pz,
+                                        unsigned ligType, float ligCharge, float4& acc,
+                                        int* cell, unsigned* npairs) {
+    acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (ligType == 0) return false;
+    if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
+          (pz >= nl.loz && pz <= nl.hiz)))
+        return false;
+    // NLGrid::getIndex (model/NLGrid.h:100-108): round half away from zero.
+    const int ix = (int)roundf((px - nl.lox) * nl.dVox_inv);
+    const int iy = (int)roundf((py - nl.loy) * nl.dVox_inv);
+    const int iz = (int)roundf((pz - nl.loz) * nl.dVox_inv);
+    if (cell) { cell[0] = ix; cell[1] = iy; cell[2] = iz; }
+    const uint2 d = nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
+    float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f;
+    unsigned pairs = 0;
+    const unsigned lt1 = ligType - 1;
+    for (unsigned j = 0; j < d.x; ++j) {
+        const unsigned nIdx = nl.list[d.y + j];
+        const float4 rp = recPos(nIdx);
+        float dx = px - rp.x;
+        float dy = py - rp.y;
+        float dz = pz - rp.z;
+        const float dr2 = dx * dx + dy * dy + dz * dz;
+        if (!(dr2 <= nl.P2)) continue;  // outOfPlateau (model/NLGrid.h:133-135); NaN skipped
+        ++pairs;
+        const float dr2_inv = (float)(1.0 / (double)dr2);
+        dx *= dr2_inv; dy *= dr2_inv; dz *= dr2_inv;
+        const unsigned rt = rec.type[nIdx];
+        // A listed bead without a valid type would index outside the table
+        // (the reference reads out of bounds there); such pairs are skipped.
+        if (rt - 1u >= par.T) continue;
+        const att_pair_params p = tab[par.T * (rt - 1) + lt1];
+        float vx, vy, vz, ve;
+        float rfx = 0.f, rfy = 0.f, rfz = 0.f;
+        lj(dr2, dr2_inv, dx, dy, dz, p, par.shape, vx, vy, vz, ve);
+        fx += vx; fy += vy; fz += vz; e += ve;
+        rfx -= vx; rfy -= vy; rfz -= vz;
+        const float qlr = ligCharge * rp.w * par.ffelec;
+        const bool calc_elec = fabsf(qlr) > 0.001f;
+        const float ratio = nl.ratio[(int)(dr2 * 10000.f)];
+        const float rdx = ratio * dx, rdy = ratio * dy, rdz = ratio * dz;
+        lj(nl.P2, nl.P2_inv, rdx, rdy, rdz, p, par.shape, vx, vy, vz, ve);
+        fx -= vx; fy -= vy; fz -= vz; e -= ve;
+        rfx += vx; rfy += vy; rfz += vz;
+        if (calc_elec) {
+            coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+            fx += vx; fy += vy; fz += vz; e += ve;
+            rfx -= vx; rfy -= vy; rfz -= vz;
+            coulomb(nl.P2_inv, rdx, rdy, rdz, qlr, par.dielec, vx, vy, vz, ve);
+            fx -= vx; fy -= vy; fz -= vz; e -= ve;
+            rfx += vx; rfy += vy; rfz += vz;
+        }
+        reaction(nIdx, rfx, rfy, rfz);
+    }
+    acc = make_float4(fx, fy, fz, e);
+    if (npairs) *npairs = pairs;
+    return true;
+}
+
+// Butterfly sum over the 64 lanes of a wave (same value on every lane).
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+// reducePotForce clamp (model/reduction.h:60-72); x *= 0.01 is a double multiply.
+__device__ __forceinline__ void force_clamp(float& x, float& y, float& z) {
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        const float magn2 = x * x + y * y + z * z;
+        if (magn2 > 1.0e18f) {
+            x = (float)((double)x * 0.01);
+            y = (float)((double)y * 0.01);
+            z = (float)((double)z * 0.01);
+        }
+    }
+}
+
+// correctModeForce (model/reduction_modes.h:100-115): d += 4 k d^4, the power in double.
+__device__ __forceinline__ float correct_mode(float d, float k) {
+    const double dd = (double)d, d2 = dd * dd;
+    const float cf = (float)((double)(4.f * k) * (d2 * d2));
+    return d + cf;
+}
+
+}  // namespace att

@@ -1,0 +1,34 @@
+// Launchers shared between the kernel translation units and the host runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace att {
+
+hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, const DGrid& outer,
+                        const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
+                        hipStream_t s);
+
+size_t modes_lds_bytes(const DProt& rec, const DPar& par, unsigned* tabSlots);
+
+hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
+                        const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
+                        const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
+                        unsigned n, hipStream_t s);
+
+hipError_t launch_debug_rigid(const DProt& rec, const DProt& lig, const DGrid& inner,
+                              const DGrid& outer, const DNL& nl, const DPar& par,
+                              const float* dof, float* xyz, float* pot, int* cell,
+                              unsigned* npairs, hipStream_t s);
+
+hipError_t launch_debug_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
+                              const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
+                              const DGrid& lOuter, const DPar& par, const float* dof, float* xyz,
+                              float* pot, int* cell, unsigned* npairs, float* rpot, float* rxyz,
+                              float* rdefo, hipStream_t s);
+
+hipError_t launch_debug_rotmats(const float* ang, unsigned n, float* R, hipStream_t s);
+
+}  // namespace att

@@ -1,0 +1,587 @@
+// Host runtime behind include/attract_hip.h: contexts, device-resident data
+// objects addressed by handles, input validation, and the scoring entry points.
+//
+// Device data layout (per object, all in HBM, uploaded once):
+//   protein  float4 {x,y,z,charge}[n], u32 type[n], u32 mappedType[n],
+//            modes [c][m][n] (coalesced over beads) and [n][m][c] (one run per
+//            bead, for neighbour-list gathers), modeForce[M]
+//   grid     inner/outer float4 [grid][z][y][x] exactly as the reference's
+//            IntrplGrid, neighbour-list {numEl, idx}[z][y][x] + u32 list, and
+//            the plateau-ratio table of model/NLGrid.tpp:44-51
+//   params   att_pair_params[T*T] + shape/dielectric/ffelec
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "attract_hip.h"
+#include "internal.h"
+
+using namespace att;
+
+namespace {
+
+enum class Kind { Protein, Grid, Params };
+
+struct Obj {
+    explicit Obj(Kind k) : kind(k) {}
+    virtual ~Obj() {
+        for (void* p : allocs) (void)hipFree(p);
+    }
+    Kind kind;
+    std::vector<void*> allocs;
+};
+
+struct ProtObj : Obj {
+    ProtObj() : Obj(Kind::Protein) {}
+    DProt d{};
+    unsigned maxType = 0, maxMType = 0;
+};
+
+struct GridObj : Obj {
+    GridObj() : Obj(Kind::Grid) {}
+    DGrid inner{}, outer{};
+    DNL nl{};
+    bool hasNL = false;
+    uint32_t nlMaxIndex = 0;
+    bool nlEmpty = true;
+};
+
+struct ParObj : Obj {
+    ParObj() : Obj(Kind::Params) {}
+    DPar d{};
+    float epsilon = 0.f;
+};
+
+}  // namespace
+
+struct att_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    uint64_t next = 1;
+    std::unordered_map<uint64_t, std::unique_ptr<Obj>> objs;
+    void* d_in = nullptr;
+    size_t d_in_cap = 0;
+    void* d_out = nullptr;
+    size_t d_out_cap = 0;
+    size_t maxLdsPerBlock = 65536;
+};
+
+namespace {
+
+att_status fail(att_ctx* c, att_status code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define ATT_HIP(ctx, expr)                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? ATT_ERR_NOMEM : ATT_ERR_HIP,         \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+template <class T>
+att_status dev_copy(att_ctx* c, Obj* o, const T* host, size_t count, T** out) {
+    *out = nullptr;
+    if (count == 0) return ATT_OK;
+    void* p = nullptr;
+    ATT_HIP(c, hipMalloc(&p, sizeof(T) * count));
+    o->allocs.push_back(p);
+    ATT_HIP(c, hipMemcpy(p, host, sizeof(T) * count, hipMemcpyHostToDevice));
+    *out = static_cast<T*>(p);
+    return ATT_OK;
+}
+
+template <class T>
+T* lookup(att_ctx* c, att_handle h, Kind k) {
+    auto it = c->objs.find(h);
+    if (it == c->objs.end() || it->second->kind != k) return nullptr;
+    return static_cast<T*>(it->second.get());
+}
+
+att_handle store(att_ctx* c, std::unique_ptr<Obj> o) {
+    const att_handle h = c->next++;
+    c->objs.emplace(h, std::move(o));
+    return h;
+}
+
+// Grid constants as IntrplGrid's constructor computes them (float arithmetic).
+DGrid make_dgrid(const att_grid_desc& g, const float4* data) {
+    DGrid d{};
+    d.data = data;
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz;
+    d.nxy = g.nx * g.ny;
+    d.nvox = g.nx * g.ny * g.nz;
+    d.ngrids = g.numGrids;
+    d.dVox = g.spacing;
+    const float vol = d.dVox * d.dVox * d.dVox;
+    d.vol_inv = 1.f / vol;
+    d.dVox_inv = 1.f / d.dVox;
+    d.lox = g.ox; d.loy = g.oy; d.loz = g.oz;
+    d.hix = g.ox + (float)(g.nx - 1) * d.dVox;
+    d.hiy = g.oy + (float)(g.ny - 1) * d.dVox;
+    d.hiz = g.oz + (float)(g.nz - 1) * d.dVox;
+    return d;
+}
+
+att_status check_grid_desc(att_ctx* c, const att_grid_desc* g, const char* what) {
+    if (!g || !g->data) return fail(c, ATT_ERR_INVALID, std::string(what) + ": missing grid data");
+    if (g->nx < 2 || g->ny < 2 || g->nz < 2 || g->numGrids < 1)
+        return fail(c, ATT_ERR_INVALID, std::string(what) + ": grid needs >= 2 points per axis and >= 1 grid");
+    if (!(g->spacing > 0.f) || !std::isfinite(g->spacing))
+        return fail(c, ATT_ERR_INVALID, std::string(what) + ": grid spacing must be positive");
+    const uint64_t total = (uint64_t)g->nx * g->ny * g->nz * g->numGrids;
+    if (total >= (1ull << 32))
+        return fail(c, ATT_ERR_UNSUPPORTED, std::string(what) + ": grid exceeds 2^32 points");
+    return ATT_OK;
+}
+
+// Checks shared by the scoring entry points: everything a kernel indexes with
+// is validated here so that no input can make a kernel read out of bounds.
+att_status check_pair(att_ctx* c, const ProtObj* rec, const ProtObj* lig, const GridObj* g,
+                      const ParObj* p) {
+    if (!g->hasNL) return fail(c, ATT_ERR_INVALID, "receptor grid has no neighbour list");
+    if (!g->nlEmpty && g->nlMaxIndex >= rec->d.n)
+        return fail(c, ATT_ERR_INVALID, "neighbour list refers to beads beyond the receptor");
+    if (lig->maxType > p->d.T)
+        return fail(c, ATT_ERR_INVALID, "ligand type exceeds the number of parameter-table types");
+    if (lig->maxMType >= g->inner.ngrids || lig->maxMType >= g->outer.ngrids)
+        return fail(c, ATT_ERR_INVALID, "ligand mapped type has no grid");
+    return ATT_OK;
+}
+
+}  // namespace
+
+namespace {
+// Scoped device buffers for the diagnostic entry points.
+struct Tmp {
+    std::vector<void*> p;
+    ~Tmp() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <class T>
+    hipError_t alloc(T** out, size_t count) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1));
+        if (e == hipSuccess) p.push_back(q);
+        *out = static_cast<T*>(q);
+        return e;
+    }
+};
+}  // namespace
+
+extern "C" {
+
+const char* att_build_info(void) {
+    return "gpuattract_2.0_amd: gfx950, -O3 -ffp-contract=off, packed-fp32 disabled";
+}
+
+att_status att_ctx_create(int hip_device, att_ctx** out) {
+    if (!out) return ATT_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return ATT_ERR_HIP;
+    if (hip_device < 0 || hip_device >= count) return ATT_ERR_INVALID;
+    if (hipSetDevice(hip_device) != hipSuccess) return ATT_ERR_HIP;
+    auto c = std::make_unique<att_ctx>();
+    c->device = hip_device;
+    if (hipStreamCreate(&c->stream) != hipSuccess) return ATT_ERR_HIP;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, hip_device) == hipSuccess && lds > 0)
+        c->maxLdsPerBlock = (size_t)lds;
+    *out = c.release();
+    return ATT_OK;
+}
+
+void att_ctx_destroy(att_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->objs.clear();
+    if (c->d_in) (void)hipFree(c->d_in);
+    if (c->d_out) (void)hipFree(c->d_out);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* att_last_error(const att_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+att_status att_upload_protein(att_ctx* c, const att_protein_desc* d, att_handle* out) {
+    if (!c || !d || !out) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    if (d->n == 0) return fail(c, ATT_ERR_INVALID, "protein has no beads");
+    if (d->numModes > ATT_MODES_MAX) return fail(c, ATT_ERR_INVALID, "more than 10 modes");
+    if (!d->xyz || !d->type || !d->mappedType || !d->charge)
+        return fail(c, ATT_ERR_INVALID, "protein arrays missing");
+    if (d->numModes > 0 && (!d->modes || !d->modeForce))
+        return fail(c, ATT_ERR_INVALID, "mode arrays missing");
+    const unsigned n = d->n, M = d->numModes;
+    auto o = std::make_unique<ProtObj>();
+    std::vector<float4> pos(n);
+    for (unsigned i = 0; i < n; ++i)
+        pos[i] = make_float4(d->xyz[i], d->xyz[n + i], d->xyz[2 * (size_t)n + i], d->charge[i]);
+    o->maxType = *std::max_element(d->type, d->type + n);
+    o->maxMType = *std::max_element(d->mappedType, d->mappedType + n);
+    float4* dpos;
+    unsigned *dtype, *dmtype;
+    att_status st;
+    if ((st = dev_copy(c, o.get(), pos.data(), n, &dpos)) != ATT_OK) return st;
+    if ((st = dev_copy(c, o.get(), d->type, n, &dtype)) != ATT_OK) return st;
+    if ((st = dev_copy(c, o.get(), d->mappedType, n, &dmtype)) != ATT_OK) return st;
+    o->d.pos = dpos;
+    o->d.type = dtype;
+    o->d.mtype = dmtype;
+    o->d.n = n;
+    o->d.M = M;
+    if (M > 0) {
+        const size_t nM = (size_t)n * M;
+        std::vector<float> soa(3 * nM), aos(3 * nM);
+        for (unsigned cc = 0; cc < 3; ++cc)
+            for (unsigned i = 0; i < n; ++i)
+                for (unsigned m = 0; m < M; ++m) {
+                    const float v = d->modes[cc * nM + (size_t)i * M + m];
+                    soa[(cc * M + m) * (size_t)n + i] = v;
+                    aos[((size_t)i * M + m) * 3 + cc] = v;
+                }
+        float *dsoa, *daos, *dmf;
+        if ((st = dev_copy(c, o.get(), soa.data(), soa.size(), &dsoa)) != ATT_OK) return st;
+        if ((st = dev_copy(c, o.get(), aos.data(), aos.size(), &daos)) != ATT_OK) return st;
+        if ((st = dev_copy(c, o.get(), d->modeForce, M, &dmf)) != ATT_OK) return st;
+        o->d.modes = dsoa;
+        o->d.modesT = daos;
+        o->d.modeForce = dmf;
+    }
+    *out = store(c, std::move(o));
+    return ATT_OK;
+}
+
+att_status att_upload_grid(att_ctx* c, const att_grid_desc* inner, const att_grid_desc* outer,
+                           const att_nlgrid_desc* nl, att_handle* out) {
+    if (!c || !out) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_status st;
+    if ((st = check_grid_desc(c, inner, "inner")) != ATT_OK) return st;
+    if ((st = check_grid_desc(c, outer, "outer")) != ATT_OK) return st;
+    if (inner->numGrids != outer->numGrids)
+        return fail(c, ATT_ERR_INVALID, "inner and outer grids differ in the number of types");
+    auto o = std::make_unique<GridObj>();
+    float4 *din, *dout;
+    const size_t nin = (size_t)inner->nx * inner->ny * inner->nz * inner->numGrids;
+    const size_t nout = (size_t)outer->nx * outer->ny * outer->nz * outer->numGrids;
+    if ((st = dev_copy(c, o.get(), reinterpret_cast<const float4*>(inner->data), nin, &din)) != ATT_OK) return st;
+    if ((st = dev_copy(c, o.get(), reinterpret_cast<const float4*>(outer->data), nout, &dout)) != ATT_OK) return st;
+    o->inner = make_dgrid(*inner, din);
+    o->outer = make_dgrid(*outer, dout);
+    if (nl) {
+        if (!nl->desc) return fail(c, ATT_ERR_INVALID, "neighbour grid descriptors missing");
+        if (nl->nx < 1 || nl->ny < 1 || nl->nz < 1) return fail(c, ATT_ERR_INVALID, "empty neighbour grid");
+        if (!(nl->plateau > 0.f) || !(nl->spacing > 0.f))
+            return fail(c, ATT_ERR_INVALID, "neighbour grid needs positive spacing and plateau");
+        const size_t nv = (size_t)nl->nx * nl->ny * nl->nz;
+        if (nv >= (1ull << 31)) return fail(c, ATT_ERR_UNSUPPORTED, "neighbour grid too large");
+        if (nl->numEl > 0 && !nl->list) return fail(c, ATT_ERR_INVALID, "neighbour list missing");
+        if (nl->numEl >= (1ull << 32)) return fail(c, ATT_ERR_UNSUPPORTED, "neighbour list exceeds 2^32 entries");
+        // Every voxel's list must lie inside the concatenated array.
+        for (size_t v = 0; v < nv; ++v) {
+            const uint64_t cnt = nl->desc[2 * v], idx = nl->desc[2 * v + 1];
+            if (cnt > 0 && idx + cnt > nl->numEl)
+                return fail(c, ATT_ERR_INVALID, "neighbour list descriptor points outside the list");
+        }
+        uint32_t mx = 0;
+        for (size_t k = 0; k < nl->numEl; ++k) mx = std::max(mx, nl->list[k]);
+        o->nlMaxIndex = mx;
+        o->nlEmpty = nl->numEl == 0;
+        uint2* ddesc;
+        uint32_t* dlist = nullptr;
+        if ((st = dev_copy(c, o.get(), reinterpret_cast<const uint2*>(nl->desc), nv, &ddesc)) != ATT_OK) return st;
+        if (nl->numEl > 0 && (st = dev_copy(c, o.get(), nl->list, nl->numEl, &dlist)) != ATT_OK) return st;
+        // Plateau ratios, model/NLGrid.tpp:44-51.
+        const float P2 = nl->plateau * nl->plateau;
+        const int size = (int)(10000 * P2);
+        std::vector<float> ratio((size_t)size + 1);
+        for (int k = 0; k <= size; ++k) {
+            const double d2 = ((k + 0.5) / 10000);
+            ratio[k] = (float)std::sqrt(d2 / (double)P2);
+        }
+        float* dratio;
+        if ((st = dev_copy(c, o.get(), ratio.data(), ratio.size(), &dratio)) != ATT_OK) return st;
+        DNL& d = o->nl;
+        d.desc = ddesc;
+        d.list = dlist;
+        d.ratio = dratio;
+        d.nx = nl->nx; d.ny = nl->ny; d.nz = nl->nz;
+        const float dv = nl->spacing;
+        d.lox = nl->ox; d.loy = nl->oy; d.loz = nl->oz;
+        d.hix = nl->ox + (float)(nl->nx - 1) * dv;
+        d.hiy = nl->oy + (float)(nl->ny - 1) * dv;
+        d.hiz = nl->oz + (float)(nl->nz - 1) * dv;
+        d.dVox_inv = 1.f / dv;
+        d.P2 = P2;
+        d.P2_inv = 1.f / P2;
+        o->hasNL = true;
+    }
+    *out = store(c, std::move(o));
+    return ATT_OK;
+}
+
+att_status att_upload_params(att_ctx* c, const att_pair_params* table, unsigned numTypes,
+                             int potShape, int dielec, float epsilon, float ffelec,
+                             att_handle* out) {
+    if (!c || !table || !out) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    if (numTypes == 0 || numTypes > 99) return fail(c, ATT_ERR_INVALID, "number of types must be 1..99");
+    if (potShape != 8 && potShape != 12) return fail(c, ATT_ERR_INVALID, "potential shape must be 8 or 12");
+    if (dielec != 0 && dielec != 1) return fail(c, ATT_ERR_INVALID, "dielectric must be 0 (constant) or 1 (variable)");
+    auto o = std::make_unique<ParObj>();
+    att_pair_params* dt;
+    att_status st;
+    if ((st = dev_copy(c, o.get(), table, (size_t)numTypes * numTypes, &dt)) != ATT_OK) return st;
+    o->d.tab = dt;
+    o->d.T = numTypes;
+    o->d.shape = potShape;
+    o->d.dielec = dielec;
+    o->d.ffelec = ffelec;
+    o->epsilon = epsilon;
+    *out = store(c, std::move(o));
+    return ATT_OK;
+}
+
+att_status att_release(att_ctx* c, att_handle h) {
+    if (!c) return ATT_ERR_INVALID;
+    auto it = c->objs.find(h);
+    if (it == c->objs.end()) return fail(c, ATT_ERR_HANDLE, "unknown handle");
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->objs.erase(it);
+    return ATT_OK;
+}
+
+static att_status resolve(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                          ProtObj** rec, ProtObj** lig, GridObj** g, ParObj** p) {
+    *rec = lookup<ProtObj>(c, hr, Kind::Protein);
+    *lig = lookup<ProtObj>(c, hl, Kind::Protein);
+    *g = lookup<GridObj>(c, hg, Kind::Grid);
+    *p = lookup<ParObj>(c, hp, Kind::Params);
+    if (!*rec || !*lig || !*g || !*p) return fail(c, ATT_ERR_HANDLE, "unknown handle or wrong kind");
+    return ATT_OK;
+}
+
+static att_status ensure_staging(att_ctx* c, size_t inBytes, size_t outBytes) {
+    if (inBytes > c->d_in_cap) {
+        if (c->d_in) ATT_HIP(c, hipFree(c->d_in));
+        c->d_in = nullptr;
+        c->d_in_cap = 0;
+        ATT_HIP(c, hipMalloc(&c->d_in, inBytes));
+        c->d_in_cap = inBytes;
+    }
+    if (outBytes > c->d_out_cap) {
+        if (c->d_out) ATT_HIP(c, hipFree(c->d_out));
+        c->d_out = nullptr;
+        c->d_out_cap = 0;
+        ATT_HIP(c, hipMalloc(&c->d_out, outBytes));
+        c->d_out_cap = outBytes;
+    }
+    return ATT_OK;
+}
+
+att_status att_score_rigid_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle hg,
+                               att_handle hp, const att_dof6* d_dofs, size_t n,
+                               att_result6* d_out, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (n == 0) return ATT_OK;
+    if (!d_dofs || !d_out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    if (n > 0xFFFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one call");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* g;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
+    if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, launch_rigid(rec->d, lig->d, g->inner, g->outer, g->nl, p->d,
+                            reinterpret_cast<const float*>(d_dofs), reinterpret_cast<float*>(d_out),
+                            (unsigned)n, s));
+    return ATT_OK;
+}
+
+att_status att_score_rigid(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                           const att_dof6* dofs, size_t n, att_result6* out, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (n == 0) return ATT_OK;
+    if (!dofs || !out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_status st;
+    if ((st = ensure_staging(c, n * sizeof(att_dof6), n * sizeof(att_result6))) != ATT_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, hipMemcpyAsync(c->d_in, dofs, n * sizeof(att_dof6), hipMemcpyHostToDevice, s));
+    if ((st = att_score_rigid_dev(c, hr, hl, hg, hp, static_cast<const att_dof6*>(c->d_in), n,
+                                  static_cast<att_result6*>(c->d_out), s)) != ATT_OK)
+        return st;
+    ATT_HIP(c, hipMemcpyAsync(out, c->d_out, n * sizeof(att_result6), hipMemcpyDeviceToHost, s));
+    ATT_HIP(c, hipStreamSynchronize(s));
+    return ATT_OK;
+}
+
+static att_status check_modes(att_ctx* c, ProtObj* rec, ProtObj* lig, GridObj* gr, GridObj* gl,
+                              ParObj* p) {
+    att_status st;
+    if ((st = check_pair(c, rec, lig, gr, p)) != ATT_OK) return st;
+    if (rec->maxMType >= gl->inner.ngrids || rec->maxMType >= gl->outer.ngrids)
+        return fail(c, ATT_ERR_INVALID, "receptor mapped type has no ligand grid");
+    const size_t lds = modes_lds_bytes(rec->d, p->d, nullptr) + sizeof(float) * 4 * 33;
+    if (lds > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "receptor too large for the LDS-staged modes kernel");
+    return ATT_OK;
+}
+
+att_status att_score_modes_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle hgr,
+                               att_handle hgl, att_handle hp, const att_dof6m* d_dofs, size_t n,
+                               att_result6m* d_out, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (n == 0) return ATT_OK;
+    if (!d_dofs || !d_out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    if (n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one call");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* gr;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hgr, hp, &rec, &lig, &gr, &p)) != ATT_OK) return st;
+    GridObj* gl = lookup<GridObj>(c, hgl, Kind::Grid);
+    if (!gl) return fail(c, ATT_ERR_HANDLE, "unknown ligand grid handle");
+    if ((st = check_modes(c, rec, lig, gr, gl, p)) != ATT_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, launch_modes(rec->d, lig->d, gr->inner, gr->outer, gr->nl, gl->inner, gl->outer,
+                            p->d, reinterpret_cast<const float*>(d_dofs),
+                            reinterpret_cast<float*>(d_out), (unsigned)n, s));
+    return ATT_OK;
+}
+
+att_status att_score_modes(att_ctx* c, att_handle hr, att_handle hl, att_handle hgr, att_handle hgl,
+                           att_handle hp, const att_dof6m* dofs, size_t n, att_result6m* out,
+                           void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (n == 0) return ATT_OK;
+    if (!dofs || !out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_status st;
+    if ((st = ensure_staging(c, n * sizeof(att_dof6m), n * sizeof(att_result6m))) != ATT_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, hipMemcpyAsync(c->d_in, dofs, n * sizeof(att_dof6m), hipMemcpyHostToDevice, s));
+    if ((st = att_score_modes_dev(c, hr, hl, hgr, hgl, hp, static_cast<const att_dof6m*>(c->d_in), n,
+                                  static_cast<att_result6m*>(c->d_out), s)) != ATT_OK)
+        return st;
+    ATT_HIP(c, hipMemcpyAsync(out, c->d_out, n * sizeof(att_result6m), hipMemcpyDeviceToHost, s));
+    ATT_HIP(c, hipStreamSynchronize(s));
+    return ATT_OK;
+}
+
+
+att_status att_debug_beads_rigid(att_ctx* c, att_handle hr, att_handle hl, att_handle hg,
+                                 att_handle hp, const att_dof6* dof, float* xyz, float* pot,
+                                 int* cell, unsigned* nl_pairs) {
+    if (!c || !dof || !xyz || !pot || !cell || !nl_pairs) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* g;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
+    if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
+    const unsigned n = lig->d.n;
+    Tmp t;
+    float *ddof, *dxyz, *dpot;
+    int* dcell;
+    unsigned* dnp;
+    ATT_HIP(c, t.alloc(&ddof, 6));
+    ATT_HIP(c, t.alloc(&dxyz, 3 * (size_t)n));
+    ATT_HIP(c, t.alloc(&dpot, 4 * (size_t)n));
+    ATT_HIP(c, t.alloc(&dcell, 9 * (size_t)n));
+    ATT_HIP(c, t.alloc(&dnp, n));
+    ATT_HIP(c, hipMemcpy(ddof, dof, sizeof(att_dof6), hipMemcpyHostToDevice));
+    ATT_HIP(c, launch_debug_rigid(rec->d, lig->d, g->inner, g->outer, g->nl, p->d, ddof, dxyz, dpot,
+                                  dcell, dnp, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    ATT_HIP(c, hipMemcpy(xyz, dxyz, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(pot, dpot, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(cell, dcell, sizeof(int) * 9 * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(nl_pairs, dnp, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    return ATT_OK;
+}
+
+att_status att_debug_beads_modes(att_ctx* c, att_handle hr, att_handle hl, att_handle hgr,
+                                 att_handle hgl, att_handle hp, const att_dof6m* dof, float* xyz,
+                                 float* pot, int* cell, unsigned* nl_pairs, float* rec_pot,
+                                 float* rec_xyz) {
+    if (!c || !dof || !xyz || !pot || !cell || !nl_pairs || !rec_pot || !rec_xyz)
+        return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* gr;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hgr, hp, &rec, &lig, &gr, &p)) != ATT_OK) return st;
+    GridObj* gl = lookup<GridObj>(c, hgl, Kind::Grid);
+    if (!gl) return fail(c, ATT_ERR_HANDLE, "unknown ligand grid handle");
+    if ((st = check_pair(c, rec, lig, gr, p)) != ATT_OK) return st;
+    if (rec->maxMType >= gl->inner.ngrids || rec->maxMType >= gl->outer.ngrids)
+        return fail(c, ATT_ERR_INVALID, "receptor mapped type has no ligand grid");
+    const unsigned n = lig->d.n, nr = rec->d.n;
+    Tmp t;
+    float *ddof, *dxyz, *dpot, *drpot, *drxyz, *drdefo;
+    int* dcell;
+    unsigned* dnp;
+    ATT_HIP(c, t.alloc(&ddof, 26));
+    ATT_HIP(c, t.alloc(&dxyz, 3 * (size_t)n));
+    ATT_HIP(c, t.alloc(&dpot, 4 * (size_t)n));
+    ATT_HIP(c, t.alloc(&dcell, 9 * (size_t)n));
+    ATT_HIP(c, t.alloc(&dnp, n));
+    ATT_HIP(c, t.alloc(&drpot, 3 * (size_t)nr));
+    ATT_HIP(c, t.alloc(&drxyz, 3 * (size_t)nr));
+    ATT_HIP(c, t.alloc(&drdefo, 3 * (size_t)nr));
+    ATT_HIP(c, hipMemcpy(ddof, dof, sizeof(att_dof6m), hipMemcpyHostToDevice));
+    ATT_HIP(c, launch_debug_modes(rec->d, lig->d, gr->inner, gr->outer, gr->nl, gl->inner, gl->outer,
+                                  p->d, ddof, dxyz, dpot, dcell, dnp, drpot, drxyz, drdefo, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    ATT_HIP(c, hipMemcpy(xyz, dxyz, sizeof(float) * 3 * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(pot, dpot, sizeof(float) * 4 * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(cell, dcell, sizeof(int) * 9 * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(nl_pairs, dnp, sizeof(unsigned) * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(rec_pot, drpot, sizeof(float) * 3 * nr, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(rec_xyz, drxyz, sizeof(float) * 3 * nr, hipMemcpyDeviceToHost));
+    return ATT_OK;
+}
+
+att_status att_debug_rotmats(att_ctx* c, const float* angles, size_t n, float* R) {
+    if (!c || (n && (!angles || !R))) return fail(c, ATT_ERR_INVALID, "null argument");
+    if (n == 0) return ATT_OK;
+    if (n > (1u << 30)) return fail(c, ATT_ERR_UNSUPPORTED, "too many angle triples");
+    ATT_HIP(c, hipSetDevice(c->device));
+    Tmp t;
+    float *da, *dR;
+    ATT_HIP(c, t.alloc(&da, 3 * n));
+    ATT_HIP(c, t.alloc(&dR, 9 * n));
+    ATT_HIP(c, hipMemcpy(da, angles, sizeof(float) * 3 * n, hipMemcpyHostToDevice));
+    ATT_HIP(c, launch_debug_rotmats(da, (unsigned)n, dR, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    ATT_HIP(c, hipMemcpy(R, dR, sizeof(float) * 9 * n, hipMemcpyDeviceToHost));
+    return ATT_OK;
+}
+
+att_status att_sync(att_ctx* c, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipStreamSynchronize(stream ? (hipStream_t)stream : c->stream));
+    return ATT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,394 @@
+// Scoring kernels for gfx950 and their launchers.
+//
+//  k_rigid  one 64-lane wavefront per pose, four poses per 256-thread workgroup.
+//           Lanes stride over ligand beads: transform, grid interpolation,
+//           neighbour-list term, and per-lane partial sums of E, F and the
+//           torque tensor; a butterfly reduction and a one-lane epilogue
+//           (force clamp, torque matrix) write the 28-byte result.  Replaces
+//           d_DOF2Pos, d_potForce, d_NLPotForce, d_reduce and h_finalReduce
+//           (model/*.cu, model/reduction.h:159-207) with one launch and no
+//           intermediate buffers.
+//  k_modes  one 256-thread workgroup per pose.  Phase A moves the deformed
+//           receptor into the ligand frame, scores it on the ligand grid and
+//           keeps its deformed coordinates in LDS; phase B scores the ligand on
+//           the receptor grid and runs the neighbour list against those LDS
+//           coordinates, folding the receptor reaction straight into the
+//           receptor mode gradients.  Semantics of
+//           service/CPUEnergyService6DModes.tpp:121-327.
+#include "device.h"
+#include "internal.h"
+
+namespace att {
+
+// Pair table into LDS when it fits (kMaxTypesLDS), else read from global.
+__device__ __forceinline__ const att_pair_params* stage_table(const DPar& par,
+                                                              att_pair_params* s_tab) {
+    if (par.T <= kMaxTypesLDS) {
+        for (unsigned k = threadIdx.x; k < par.T * par.T; k += blockDim.x) s_tab[k] = par.tab[k];
+        return s_tab;
+    }
+    return par.tab;
+}
+
+struct NoReaction {
+    __device__ void operator()(unsigned, float, float, float) const {}
+};
+
+__global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer,
+                                               DNL nl, DPar par, const float* __restrict__ dofs,
+                                               float* __restrict__ res, unsigned n) {
+    extern __shared__ att_pair_params s_tab[];
+    const att_pair_params* tab = stage_table(par, s_tab);
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned pose = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (pose >= n) return;
+    const float* d = dofs + 6 * (size_t)pose;
+    const float tx = d[0], ty = d[1], tz = d[2];
+    float t[6];
+    pose_trig(d[3], d[4], d[5], lane, t);
+    float R[9];
+    rotmat_from_trig(t[0], t[1], t[2], t[3], t[4], t[5], R);
+    float acc[13];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) acc[k] = 0.f;
+    const float4* __restrict__ rpos = rec.pos;
+    for (unsigned i = lane; i < lig.n; i += kWave) {
+        const float4 x = lig.pos[i];
+        V3 p = matvec(R, x.x, x.y, x.z);
+        p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
+        float4 pot = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
+        float4 a;
+        if (nl_bead(nl, rec, par, tab, [rpos](unsigned j) { return rpos[j]; }, NoReaction{}, p.x,
+                    p.y, p.z, lig.type[i], x.w, a, nullptr, nullptr)) {
+            pot.x = pot.x + a.x; pot.y = pot.y + a.y; pot.z = pot.z + a.z; pot.w = pot.w + a.w;
+        }
+        acc[0] += pot.w;
+        acc[1] += pot.x; acc[2] += pot.y; acc[3] += pot.z;
+        acc[4] += x.x * pot.x; acc[5] += x.y * pot.x; acc[6] += x.z * pot.x;
+        acc[7] += x.x * pot.y; acc[8] += x.y * pot.y; acc[9] += x.z * pot.y;
+        acc[10] += x.x * pot.z; acc[11] += x.y * pot.z; acc[12] += x.z * pot.z;
+    }
+#pragma unroll
+    for (int k = 0; k < 13; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0) {
+        float Fx = acc[1], Fy = acc[2], Fz = acc[3];
+        force_clamp(Fx, Fy, Fz);
+        float a3[3];
+        torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], acc + 4, a3);
+        float* o = res + 7 * (size_t)pose;
+        o[0] = acc[0]; o[1] = Fx; o[2] = Fy; o[3] = Fz;
+        o[4] = a3[0]; o[5] = a3[1]; o[6] = a3[2];
+    }
+}
+
+// Mode deformation x + sum_m a_m * mode_m (model/transform.h:69-73), modes
+// stored [c][m][n].
+__device__ __forceinline__ V3 deform(const DProt& b, unsigned i, float x, float y, float z,
+                                     const float* amp) {
+    V3 p{x, y, z};
+    const size_t nM = (size_t)b.M * b.n;
+#pragma unroll
+    for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
+        if (m < b.M) {
+            const float a = amp[m];
+            p.x = p.x + a * b.modes[(size_t)m * b.n + i];
+            p.y = p.y + a * b.modes[nM + (size_t)m * b.n + i];
+            p.z = p.z + a * b.modes[2 * nM + (size_t)m * b.n + i];
+        }
+    }
+    return p;
+}
+
+constexpr int kModesVals = 13 + 2 * ATT_MODES_MAX;
+
+__global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter,
+                                               DNL nl, DGrid lInner, DGrid lOuter, DPar par,
+                                               const float* __restrict__ dofs,
+                                               float* __restrict__ res, unsigned n,
+                                               unsigned tabSlots) {
+    extern __shared__ att_pair_params s_dyn[];
+    __shared__ float s_red[4][kModesVals];
+    const att_pair_params* tab = stage_table(par, s_dyn);
+    float* s_rx = reinterpret_cast<float*>(s_dyn + tabSlots);
+    float* s_ry = s_rx + rec.n;
+    float* s_rz = s_ry + rec.n;
+    const unsigned pose = blockIdx.x;
+    if (pose >= n) return;  // uniform per workgroup
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const float* d = dofs + 26 * (size_t)pose;
+    const float tx = d[0], ty = d[1], tz = d[2];
+    float ampR[ATT_MODES_MAX], ampL[ATT_MODES_MAX];
+#pragma unroll
+    for (int m = 0; m < ATT_MODES_MAX; ++m) { ampR[m] = d[6 + m]; ampL[m] = d[16 + m]; }
+    float t[6];
+    pose_trig(d[3], d[4], d[5], lane, t);
+    float R[9], RT[9], Rinv[9];
+    rotmat_from_trig(t[0], t[1], t[2], t[3], t[4], t[5], R);
+    RT[0] = R[0]; RT[1] = R[3]; RT[2] = R[6];
+    RT[3] = R[1]; RT[4] = R[4]; RT[5] = R[7];
+    RT[6] = R[2]; RT[7] = R[5]; RT[8] = R[8];
+    // invertDOF (model/transform_modes.h:26-41): rotation R(-ang) (cos even,
+    // sin odd), translation R^T (-pos).
+    rotmat_from_trig(t[0], t[1], -t[2], -t[3], t[4], -t[5], Rinv);
+    const V3 pinv = matvec(RT, -tx, -ty, -tz);
+
+    float acc[13];
+    float mR[ATT_MODES_MAX], mL[ATT_MODES_MAX];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) acc[k] = 0.f;
+#pragma unroll
+    for (int m = 0; m < ATT_MODES_MAX; ++m) { mR[m] = 0.f; mL[m] = 0.f; }
+
+    // Phase A: receptor beads.
+    const size_t rnM = (size_t)rec.M * rec.n;
+    for (unsigned j = threadIdx.x; j < rec.n; j += blockDim.x) {
+        const float4 x = rec.pos[j];
+        const V3 df = deform(rec, j, x.x, x.y, x.z, ampR);
+        s_rx[j] = df.x; s_ry[j] = df.y; s_rz[j] = df.z;
+        V3 p = matvec(Rinv, df.x, df.y, df.z);
+        p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
+        const float4 pot = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
+        const V3 f = matvec(R, pot.x, pot.y, pot.z);  // rotate_forces (transform_modes.h:141-161)
+#pragma unroll
+        for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
+            if (m < rec.M)
+                mR[m] -= f.x * rec.modes[(size_t)m * rec.n + j] +
+                         f.y * rec.modes[rnM + (size_t)m * rec.n + j] +
+                         f.z * rec.modes[2 * rnM + (size_t)m * rec.n + j];
+    }
+    __syncthreads();
+
+    // Phase B: ligand beads.
+    const float4* __restrict__ rpos = rec.pos;
+    const float* __restrict__ rmodesT = rec.modesT;
+    const unsigned Mr = rec.M;
+    auto recPos = [=](unsigned j) { return make_float4(s_rx[j], s_ry[j], s_rz[j], rpos[j].w); };
+    auto reaction = [&](unsigned j, float fx, float fy, float fz) {
+        const float* mt = rmodesT + (size_t)j * Mr * 3;
+#pragma unroll
+        for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
+            if (m < Mr) mR[m] -= fx * mt[3 * m] + fy * mt[3 * m + 1] + fz * mt[3 * m + 2];
+    };
+    const size_t lnM = (size_t)lig.M * lig.n;
+    for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
+        const float4 x = lig.pos[i];
+        const V3 df = deform(lig, i, x.x, x.y, x.z, ampL);
+        V3 p = matvec(R, df.x, df.y, df.z);
+        p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
+        float4 pot = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
+        float4 a;
+        if (nl_bead(nl, rec, par, tab, recPos, reaction, p.x, p.y, p.z, lig.type[i], x.w, a,
+                    nullptr, nullptr)) {
+            pot.x = pot.x + a.x; pot.y = pot.y + a.y; pot.z = pot.z + a.z; pot.w = pot.w + a.w;
+        }
+        acc[0] += pot.w;
+        acc[1] += pot.x; acc[2] += pot.y; acc[3] += pot.z;
+        acc[4] += x.x * pot.x; acc[5] += x.y * pot.x; acc[6] += x.z * pot.x;
+        acc[7] += x.x * pot.y; acc[8] += x.y * pot.y; acc[9] += x.z * pot.y;
+        acc[10] += x.x * pot.z; acc[11] += x.y * pot.z; acc[12] += x.z * pot.z;
+        // reduceModeForce, ligand (model/reduction_modes.h:28-57): R^T f . mode.
+        const V3 fl = matvec(RT, pot.x, pot.y, pot.z);
+#pragma unroll
+        for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
+            if (m < lig.M)
+                mL[m] -= fl.x * lig.modes[(size_t)m * lig.n + i] +
+                         fl.y * lig.modes[lnM + (size_t)m * lig.n + i] +
+                         fl.z * lig.modes[2 * lnM + (size_t)m * lig.n + i];
+    }
+
+    // Workgroup reduction: wave butterflies, then the four waves in order.
+#pragma unroll
+    for (int k = 0; k < 13; ++k) acc[k] = wave_sum(acc[k]);
+#pragma unroll
+    for (int m = 0; m < ATT_MODES_MAX; ++m) { mR[m] = wave_sum(mR[m]); mL[m] = wave_sum(mL[m]); }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < 13; ++k) s_red[wave][k] = acc[k];
+#pragma unroll
+        for (int m = 0; m < ATT_MODES_MAX; ++m) {
+            s_red[wave][13 + m] = mR[m];
+            s_red[wave][13 + ATT_MODES_MAX + m] = mL[m];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v[kModesVals];
+        const int nw = blockDim.x / kWave;
+#pragma unroll
+        for (int k = 0; k < kModesVals; ++k) {
+            float s = s_red[0][k];
+            for (int w = 1; w < nw; ++w) s += s_red[w][k];
+            v[k] = s;
+        }
+        float Fx = v[1], Fy = v[2], Fz = v[3];
+        force_clamp(Fx, Fy, Fz);
+        float a3[3];
+        torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], v + 4, a3);
+        float* o = res + 27 * (size_t)pose;
+        o[0] = v[0]; o[1] = Fx; o[2] = Fy; o[3] = Fz;
+        o[4] = a3[0]; o[5] = a3[1]; o[6] = a3[2];
+        for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
+            o[7 + m] = m < rec.M ? correct_mode(v[13 + m], rec.modeForce[m]) : 0.f;
+            o[17 + m] = m < lig.M ? correct_mode(v[13 + ATT_MODES_MAX + m], lig.modeForce[m]) : 0.f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- diagnostics
+
+__global__ void k_debug_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer, DNL nl, DPar par,
+                              const float* dof, float* xyz, float* pot, int* cell,
+                              unsigned* npairs) {
+    const int lane = threadIdx.x;
+    float t[6];
+    pose_trig(dof[3], dof[4], dof[5], lane, t);
+    float R[9];
+    rotmat_from_trig(t[0], t[1], t[2], t[3], t[4], t[5], R);
+    const unsigned n = lig.n;
+    for (unsigned i = lane; i < n; i += kWave) {
+        const float4 x = lig.pos[i];
+        V3 p = matvec(R, x.x, x.y, x.z);
+        p.x = p.x + dof[0]; p.y = p.y + dof[1]; p.z = p.z + dof[2];
+        int* c = cell + 9 * i;
+        for (int k = 0; k < 9; ++k) c[k] = -1;
+        float4 pp = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, c);
+        float4 a;
+        unsigned np = 0;
+        if (nl_bead(nl, rec, par, par.tab, [&](unsigned j) { return rec.pos[j]; }, NoReaction{},
+                    p.x, p.y, p.z, lig.type[i], x.w, a, c + 6, &np)) {
+            pp.x = pp.x + a.x; pp.y = pp.y + a.y; pp.z = pp.z + a.z; pp.w = pp.w + a.w;
+        }
+        xyz[i] = p.x; xyz[n + i] = p.y; xyz[2 * n + i] = p.z;
+        pot[i] = pp.x; pot[n + i] = pp.y; pot[2 * n + i] = pp.z; pot[3 * n + i] = pp.w;
+        npairs[i] = np;
+    }
+}
+
+// One wave; the neighbour-list phase runs on lane 0 over the ligand beads in
+// order so that receptor reaction forces accumulate in the CPU's order.
+__global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, DNL nl,
+                              DGrid lInner, DGrid lOuter, DPar par, const float* d, float* xyz,
+                              float* pot, int* cell, unsigned* npairs, float* rpot, float* rxyz,
+                              float* rdefo) {
+    const int lane = threadIdx.x;
+    float t[6];
+    pose_trig(d[3], d[4], d[5], lane, t);
+    float R[9], RT[9], Rinv[9];
+    rotmat_from_trig(t[0], t[1], t[2], t[3], t[4], t[5], R);
+    RT[0] = R[0]; RT[1] = R[3]; RT[2] = R[6];
+    RT[3] = R[1]; RT[4] = R[4]; RT[5] = R[7];
+    RT[6] = R[2]; RT[7] = R[5]; RT[8] = R[8];
+    rotmat_from_trig(t[0], t[1], -t[2], -t[3], t[4], -t[5], Rinv);
+    const V3 pinv = matvec(RT, -d[0], -d[1], -d[2]);
+    const unsigned nr = rec.n, nL = lig.n;
+    for (unsigned j = lane; j < nr; j += kWave) {
+        const float4 x = rec.pos[j];
+        const V3 df = deform(rec, j, x.x, x.y, x.z, d + 6);
+        rdefo[j] = df.x; rdefo[nr + j] = df.y; rdefo[2 * nr + j] = df.z;
+        V3 p = matvec(Rinv, df.x, df.y, df.z);
+        p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
+        rxyz[j] = p.x; rxyz[nr + j] = p.y; rxyz[2 * nr + j] = p.z;
+        const float4 pp = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
+        const V3 f = matvec(R, pp.x, pp.y, pp.z);
+        rpot[j] = f.x; rpot[nr + j] = f.y; rpot[2 * nr + j] = f.z;
+    }
+    for (unsigned i = lane; i < nL; i += kWave) {
+        const float4 x = lig.pos[i];
+        const V3 df = deform(lig, i, x.x, x.y, x.z, d + 16);
+        V3 p = matvec(R, df.x, df.y, df.z);
+        p.x = p.x + d[0]; p.y = p.y + d[1]; p.z = p.z + d[2];
+        xyz[i] = p.x; xyz[nL + i] = p.y; xyz[2 * nL + i] = p.z;
+        int* c = cell + 9 * i;
+        for (int k = 0; k < 9; ++k) c[k] = -1;
+        const float4 pp = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[i], x.w, c);
+        pot[i] = pp.x; pot[nL + i] = pp.y; pot[2 * nL + i] = pp.z; pot[3 * nL + i] = pp.w;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    auto recPos = [&](unsigned j) { return make_float4(rdefo[j], rdefo[nr + j], rdefo[2 * nr + j], rec.pos[j].w); };
+    auto reaction = [&](unsigned j, float fx, float fy, float fz) {
+        rpot[j] += fx; rpot[nr + j] += fy; rpot[2 * nr + j] += fz;
+    };
+    for (unsigned i = 0; i < nL; ++i) {
+        const float4 x = lig.pos[i];
+        float4 a;
+        unsigned np = 0;
+        if (nl_bead(nl, rec, par, par.tab, recPos, reaction, xyz[i], xyz[nL + i], xyz[2 * nL + i],
+                    lig.type[i], x.w, a, cell + 9 * i + 6, &np)) {
+            pot[i] += a.x; pot[nL + i] += a.y; pot[2 * nL + i] += a.z; pot[3 * nL + i] += a.w;
+        }
+        npairs[i] = np;
+    }
+}
+
+__global__ void k_debug_rotmats(const float* ang, unsigned n, float* R) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // Per-lane evaluation (not the wave-cooperative path) of the same formula.
+    double s0, c0, s1, c1, s2, c2;
+    sincos((double)ang[3 * i + 1], &s0, &c0);  // ssi
+    sincos((double)ang[3 * i + 0], &s1, &c1);  // phi
+    sincos((double)ang[3 * i + 2], &s2, &c2);  // rot
+    rotmat_from_trig((float)c0, (float)c1, (float)s0, (float)s1, (float)c2, (float)s2, R + 9 * i);
+}
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, const DGrid& outer,
+                        const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
+                        hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const unsigned posesPerBlock = 4;
+    const unsigned blocks = (n + posesPerBlock - 1) / posesPerBlock;
+    const size_t lds = par.T <= kMaxTypesLDS ? sizeof(att_pair_params) * par.T * par.T : 0;
+    hipLaunchKernelGGL(k_rigid, dim3(blocks), dim3(256), lds, s, rec, lig, inner, outer, nl, par,
+                       dofs, res, n);
+    return hipGetLastError();
+}
+
+size_t modes_lds_bytes(const DProt& rec, const DPar& par, unsigned* tabSlots) {
+    const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
+    if (tabSlots) *tabSlots = slots;
+    return sizeof(att_pair_params) * slots + sizeof(float) * 3 * (size_t)rec.n;
+}
+
+hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
+                        const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
+                        const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
+                        unsigned n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    unsigned slots = 0;
+    const size_t lds = modes_lds_bytes(rec, par, &slots);
+    hipLaunchKernelGGL(k_modes, dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, nl, lInner,
+                       lOuter, par, dofs, res, n, slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_rigid(const DProt& rec, const DProt& lig, const DGrid& inner,
+                              const DGrid& outer, const DNL& nl, const DPar& par,
+                              const float* dof, float* xyz, float* pot, int* cell,
+                              unsigned* npairs, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_rigid, dim3(1), dim3(kWave), 0, s, rec, lig, inner, outer, nl, par,
+                       dof, xyz, pot, cell, npairs);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
+                              const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
+                              const DGrid& lOuter, const DPar& par, const float* dof, float* xyz,
+                              float* pot, int* cell, unsigned* npairs, float* rpot, float* rxyz,
+                              float* rdefo, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_modes, dim3(1), dim3(kWave), 0, s, rec, lig, rInner, rOuter, nl,
+                       lInner, lOuter, par, dof, xyz, pot, cell, npairs, rpot, rxyz, rdefo);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_rotmats(const float* ang, unsigned n, float* R, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_rotmats, dim3((n + 255) / 256), dim3(256), 0, s, ang, n, R);
+    return hipGetLastError();
+}
+
+}  // namespace att

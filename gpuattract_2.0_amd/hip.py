@@ -1,0 +1,213 @@
+"""ctypes binding of libattract_hip.so (include/attract_hip.h).
+
+Host-buffer calls take numpy arrays; the *_dev calls take device pointers
+(e.g. torch tensors on cuda via .data_ptr()) and a hipStream_t (torch's
+stream handle).  Loading fails loudly when the library is missing: there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "lib", "libattract_hip.so")
+
+P = C.c_void_p
+H = C.c_uint64
+
+ATT_OK = 0
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "out of memory", 4: "bad handle",
+          5: "unsupported"}
+
+
+class ProteinDesc(C.Structure):
+    _fields_ = [("n", C.c_uint), ("numModes", C.c_uint), ("xyz", P), ("type", P),
+                ("mappedType", P), ("charge", P), ("modes", P), ("modeForce", P)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_uint), ("ny", C.c_uint), ("nz", C.c_uint), ("numGrids", C.c_uint),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("spacing", C.c_float),
+                ("data", P)]
+
+
+class NLDesc(C.Structure):
+    _fields_ = [("nx", C.c_uint), ("ny", C.c_uint), ("nz", C.c_uint),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("spacing", C.c_float),
+                ("plateau", C.c_float), ("desc", P), ("list", P), ("numEl", C.c_size_t)]
+
+
+PAIR_DTYPE = np.dtype([("rc", np.float32), ("ac", np.float32), ("ipon", np.int32),
+                       ("rmin2", np.float32), ("emin", np.float32)])
+
+_lib = None
+
+
+class AttError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} missing: the HIP extension must be built "
+                               "(__graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIBPATH)
+        L.att_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.att_ctx_destroy.argtypes = [P]
+        L.att_ctx_destroy.restype = None
+        L.att_last_error.argtypes = [P]
+        L.att_last_error.restype = C.c_char_p
+        L.att_upload_protein.argtypes = [P, C.POINTER(ProteinDesc), C.POINTER(H)]
+        L.att_upload_grid.argtypes = [P, C.POINTER(GridDesc), C.POINTER(GridDesc),
+                                      C.POINTER(NLDesc), C.POINTER(H)]
+        L.att_upload_params.argtypes = [P, P, C.c_uint, C.c_int, C.c_int, C.c_float, C.c_float,
+                                        C.POINTER(H)]
+        L.att_release.argtypes = [P, H]
+        L.att_score_rigid.argtypes = [P, H, H, H, H, P, C.c_size_t, P, P]
+        L.att_score_rigid_dev.argtypes = [P, H, H, H, H, P, C.c_size_t, P, P]
+        L.att_score_modes.argtypes = [P, H, H, H, H, H, P, C.c_size_t, P, P]
+        L.att_score_modes_dev.argtypes = [P, H, H, H, H, H, P, C.c_size_t, P, P]
+        L.att_debug_beads_rigid.argtypes = [P, H, H, H, H, P, P, P, P, P]
+        L.att_debug_beads_modes.argtypes = [P, H, H, H, H, H, P, P, P, P, P, P, P]
+        L.att_debug_rotmats.argtypes = [P, P, C.c_size_t, P]
+        L.att_sync.argtypes = [P, P]
+        L.att_build_info.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+class Context:
+    """One att_ctx on a HIP device, with helpers to upload a Workload."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        h = P()
+        st = self.L.att_ctx_create(device, C.byref(h))
+        if st != ATT_OK:
+            raise AttError(st, f"cannot create a context on HIP device {device}")
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.L.att_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st != ATT_OK:
+            raise AttError(st, self.L.att_last_error(self.h).decode())
+
+    # -- uploads ---------------------------------------------------------
+    def upload_protein(self, body):
+        d = ProteinDesc(body.n, body.M, _p(body.xyz), _p(body.type), _p(body.mtype),
+                        _p(body.charge), _p(body.modes) if body.M else None,
+                        _p(body.mode_force) if body.M else None)
+        out = H()
+        self._chk(self.L.att_upload_protein(self.h, C.byref(d), C.byref(out)))
+        return out.value
+
+    @staticmethod
+    def _gdesc(g):
+        ge = g.geom
+        return GridDesc(ge.nx, ge.ny, ge.nz, g.ngrids, ge.ox, ge.oy, ge.oz, ge.spacing, _p(g.data))
+
+    def upload_grid(self, inner, outer, nl=None):
+        gi, go = self._gdesc(inner), self._gdesc(outer)
+        nd = None
+        if nl is not None:
+            ge = nl.geom
+            nd = NLDesc(ge.nx, ge.ny, ge.nz, ge.ox, ge.oy, ge.oz, ge.spacing, nl.plateau,
+                        _p(nl.desc), _p(nl.list) if len(nl.list) else None, len(nl.list))
+        out = H()
+        self._chk(self.L.att_upload_grid(self.h, C.byref(gi), C.byref(go),
+                                         C.byref(nd) if nd is not None else None, C.byref(out)))
+        return out.value
+
+    def upload_params(self, p):
+        tab = np.zeros(p.T * p.T, PAIR_DTYPE)
+        tab["rc"], tab["ac"], tab["ipon"] = p.rc, p.ac, p.ipon
+        tab["rmin2"], tab["emin"] = p.rmin2, p.emin
+        out = H()
+        self._chk(self.L.att_upload_params(self.h, _p(tab), p.T, p.shape, p.dielec, p.epsilon,
+                                           p.ffelec, C.byref(out)))
+        return out.value
+
+    def release(self, handle):
+        self._chk(self.L.att_release(self.h, handle))
+
+    def upload_workload(self, wl):
+        """Upload everything a workload needs; returns a dict of handles."""
+        hs = dict(rec=self.upload_protein(wl.rec), lig=self.upload_protein(wl.lig),
+                  grid=self.upload_grid(wl.rec_inner, wl.rec_outer, wl.rec_nl),
+                  params=self.upload_params(wl.params))
+        if wl.lig_inner is not None:
+            hs["lig_grid"] = self.upload_grid(wl.lig_inner, wl.lig_outer, None)
+        return hs
+
+    # -- scoring ---------------------------------------------------------
+    def score_rigid(self, hs, dofs):
+        d = np.ascontiguousarray(dofs, np.float32).reshape(-1, 6)
+        out = np.zeros((d.shape[0], 7), np.float32)
+        self._chk(self.L.att_score_rigid(self.h, hs["rec"], hs["lig"], hs["grid"], hs["params"],
+                                         _p(d), d.shape[0], _p(out), None))
+        return out
+
+    def score_rigid_dev(self, hs, d_dofs, n, d_out, stream=None):
+        self._chk(self.L.att_score_rigid_dev(self.h, hs["rec"], hs["lig"], hs["grid"],
+                                             hs["params"], d_dofs, n, d_out, stream))
+
+    def score_modes(self, hs, dofs):
+        d = np.ascontiguousarray(dofs, np.float32).reshape(-1, 26)
+        out = np.zeros((d.shape[0], 27), np.float32)
+        self._chk(self.L.att_score_modes(self.h, hs["rec"], hs["lig"], hs["grid"], hs["lig_grid"],
+                                         hs["params"], _p(d), d.shape[0], _p(out), None))
+        return out
+
+    def score_modes_dev(self, hs, d_dofs, n, d_out, stream=None):
+        self._chk(self.L.att_score_modes_dev(self.h, hs["rec"], hs["lig"], hs["grid"],
+                                             hs["lig_grid"], hs["params"], d_dofs, n, d_out,
+                                             stream))
+
+    def sync(self, stream=None):
+        self._chk(self.L.att_sync(self.h, stream))
+
+    # -- diagnostics -----------------------------------------------------
+    def beads(self, hs, wl, dof):
+        n, nr = wl.lig.n, wl.rec.n
+        out = dict(xyz=np.zeros(3 * n, np.float32), pot=np.zeros(4 * n, np.float32),
+                   cell=np.full(9 * n, -1, np.int32), nl_pairs=np.zeros(n, np.uint32))
+        d = np.ascontiguousarray(dof, np.float32)
+        if wl.modes:
+            out["rec_pot"] = np.zeros(3 * nr, np.float32)
+            out["rec_xyz"] = np.zeros(3 * nr, np.float32)
+            self._chk(self.L.att_debug_beads_modes(
+                self.h, hs["rec"], hs["lig"], hs["grid"], hs["lig_grid"], hs["params"], _p(d),
+                _p(out["xyz"]), _p(out["pot"]), _p(out["cell"]), _p(out["nl_pairs"]),
+                _p(out["rec_pot"]), _p(out["rec_xyz"])))
+        else:
+            self._chk(self.L.att_debug_beads_rigid(
+                self.h, hs["rec"], hs["lig"], hs["grid"], hs["params"], _p(d), _p(out["xyz"]),
+                _p(out["pot"]), _p(out["cell"]), _p(out["nl_pairs"])))
+        out["cell"] = out["cell"].reshape(n, 9)
+        return out
+
+    def rotmats(self, angles):
+        a = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)
+        R = np.zeros((a.shape[0], 9), np.float32)
+        self._chk(self.L.att_debug_rotmats(self.h, _p(a), a.shape[0], _p(R)))
+        return R

@@ -1,0 +1,166 @@
+/*
+ * attract_hip.h — C ABI of the MI355X (gfx950) pose-scoring library.
+ *
+ * The library replaces the reference's GPU energy services
+ * (service/GPUEnergyService6D.tpp, service/GPUEnergyService6DModes.tpp) and the
+ * device-data upload they rely on (server/DeviceDataConfigurator.tpp).  All
+ * types are plain C; no exceptions cross the boundary.  Every call returns an
+ * att_status (ATT_OK = 0); att_last_error() gives the message of the last
+ * failing call on that context.
+ *
+ * Threading: one context per host thread (the reference binds one worker
+ * thread per device, service/GPUEnergyService6D.tpp:469-477).  Calls on one
+ * context must not overlap; several contexts may share a device.  `stream` is
+ * a hipStream_t (NULL = the context's own stream).
+ */
+#ifndef ATTRACT_HIP_H_
+#define ATTRACT_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct att_ctx att_ctx;
+typedef int att_status;
+typedef uint64_t att_handle;
+
+enum {
+    ATT_OK = 0,
+    ATT_ERR_INVALID = 1,     /* bad argument (the reference throws std::invalid_argument) */
+    ATT_ERR_HIP = 2,         /* HIP runtime failure (the reference's CUDA_CHECK, commons/macros.h:62-72) */
+    ATT_ERR_NOMEM = 3,
+    ATT_ERR_HANDLE = 4,      /* unknown or wrong-kind handle (DataManager::get, server/DataManager.cpp:42-49) */
+    ATT_ERR_UNSUPPORTED = 5  /* input outside what the kernels support (sizes, types) */
+};
+
+#define ATT_MODES_MAX 10 /* MODES_MAX_NUMBER, model/Types_6D_Config.h:11 */
+
+/* Protein (model/Protein.h:38-167), coordinates pivotized. */
+typedef struct {
+    unsigned n;                /* numAtoms */
+    unsigned numModes;         /* 0..ATT_MODES_MAX */
+    const float* xyz;          /* SoA [x: n][y: n][z: n] */
+    const unsigned* type;      /* raw atom types (neighbour-list term), 0 = none */
+    const unsigned* mappedType;/* grid types (interpolation term), 0 = none */
+    const float* charge;
+    const float* modes;        /* [x: n*M][y: n*M][z: n*M], element i*M+m (Protein.h:88-98) */
+    const float* modeForce;    /* numModes force constants (readFile.cpp:218-221) */
+} att_protein_desc;
+
+/* Interpolation grid (IntrplGrid::Desc, model/IntrplGrid.h:165-199). */
+typedef struct {
+    unsigned nx, ny, nz;
+    unsigned numGrids;         /* ligand-type grids + 1; grid 0 is electrostatics */
+    float ox, oy, oz;          /* position of grid point (0,0,0) */
+    float spacing;
+    const float* data;         /* numGrids*nz*ny*nx float4 {fx,fy,fz,E} */
+} att_grid_desc;
+
+/* Neighbour-list grid (NLGrid::Desc, model/NLGrid.h:146-187). */
+typedef struct {
+    unsigned nx, ny, nz;
+    float ox, oy, oz;
+    float spacing;
+    float plateau;             /* dPlateau */
+    const uint32_t* desc;      /* nz*ny*nx {numEl, idx} */
+    const uint32_t* list;      /* concatenated receptor indices */
+    size_t numEl;
+} att_nlgrid_desc;
+
+/* ParamTable::attractFFParams_t (model/ParamTable.h:26-32). */
+typedef struct {
+    float rc, ac;
+    int ipon;
+    float rmin2, emin;
+} att_pair_params;
+
+/* DOF_6D<float> / Result_6D<float> (model/Types_6D.h:31-54). */
+typedef struct { float pos[3]; float ang[3]; } att_dof6;
+typedef struct { float E; float pos[3]; float ang[3]; } att_result6;
+
+/* DOF_6D_Modes<float> / Result_6D_Modes<float> (model/Types_6D_Modes.h:33-60). */
+typedef struct { att_dof6 d6; float modesRec[ATT_MODES_MAX]; float modesLig[ATT_MODES_MAX]; } att_dof6m;
+typedef struct { att_result6 r6; float modesRec[ATT_MODES_MAX]; float modesLig[ATT_MODES_MAX]; } att_result6m;
+
+/* --- context ---------------------------------------------------------------
+ * Replaces the per-worker CUDA state of GPUEnergyService6D::Private
+ * (service/GPUEnergyService6D.tpp:94-115) and cudaSetDevice per item (:474). */
+att_status att_ctx_create(int hip_device, att_ctx** out);
+void att_ctx_destroy(att_ctx* ctx);
+const char* att_last_error(const att_ctx* ctx);
+
+/* --- device data -----------------------------------------------------------
+ * Replace DeviceDataConfigurator::attach for Protein (DeviceDataConfigurator.tpp:27-95),
+ * GridUnion (:117-326) and ParamTable/SimParam (:337-362); att_release replaces
+ * DeviceManager::detachFromDevice (server/DeviceManager.cpp:61-111). */
+att_status att_upload_protein(att_ctx* ctx, const att_protein_desc* desc, att_handle* out);
+att_status att_upload_grid(att_ctx* ctx, const att_grid_desc* inner, const att_grid_desc* outer,
+                           const att_nlgrid_desc* nl /* may be NULL (ligand grids) */,
+                           att_handle* out);
+att_status att_upload_params(att_ctx* ctx, const att_pair_params* table, unsigned numTypes,
+                             int potShape /* 8 or 12 */, int dielec /* 0 constant, 1 variable */,
+                             float epsilon, float ffelec, att_handle* out);
+att_status att_release(att_ctx* ctx, att_handle h);
+
+/* --- rigid scoring (Types_6D<float>) --------------------------------------
+ * att_score_rigid replaces one call of GPUEnergyService6D's item processor
+ * (service/GPUEnergyService6D.tpp:469-512: H2D of the DOFs, transform,
+ * interpolation, neighbour list, reduction, D2H and the host final reduce
+ * h_finalReduce, model/reduction.h:159-207).  Host buffers in and out;
+ * returns when `out` holds the n results. */
+att_status att_score_rigid(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                           att_handle params, const att_dof6* dofs, size_t n, att_result6* out,
+                           void* stream);
+/* Same computation on device-resident DOFs/results; asynchronous on `stream`. */
+att_status att_score_rigid_dev(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                               att_handle params, const att_dof6* d_dofs, size_t n,
+                               att_result6* d_out, void* stream);
+
+/* --- flexible scoring (Types_6D_Modes<float>) -----------------------------
+ * Replaces GPUEnergyService6DModes's item processor
+ * (service/GPUEnergyService6DModes.tpp:634-676) with the semantics of the
+ * reference CPU path (service/CPUEnergyService6DModes.tpp:121-327): receptor
+ * moved into the ligand frame with R(-ang) (transform_modes.h:26-41), scored
+ * on the ligand grid, forces rotated back, neighbour list on the deformed
+ * receptor, torque with undeformed ligand coordinates, correctModeForce. */
+att_status att_score_modes(att_ctx* ctx, att_handle rec, att_handle lig, att_handle recGrid,
+                           att_handle ligGrid, att_handle params, const att_dof6m* dofs,
+                           size_t n, att_result6m* out, void* stream);
+att_status att_score_modes_dev(att_ctx* ctx, att_handle rec, att_handle lig, att_handle recGrid,
+                               att_handle ligGrid, att_handle params, const att_dof6m* d_dofs,
+                               size_t n, att_result6m* d_out, void* stream);
+
+/* --- diagnostics -------------------------------------------------------------
+ * Per-bead intermediates of one pose, computed by the same device functions
+ * as the scoring kernels: transformed coordinates (SoA 3n), per-bead
+ * {fx,fy,fz,E} after grid + neighbour list (SoA 4n), cell indices per bead
+ * (inner xyz, outer xyz, neighbour-list xyz; -1 when not used) and the number
+ * of listed neighbours inside the plateau.  Mirrors the intermediate buffers
+ * d_trafoLig / d_potLig of service/GPUEnergyService6D.tpp:117-128. */
+att_status att_debug_beads_rigid(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                                 att_handle params, const att_dof6* dof, float* xyz, float* pot,
+                                 int* cell, unsigned* nl_pairs);
+/* Modes variant; additionally the receptor forces (SoA 3*n_rec, after rotation
+ * back and neighbour-list reaction) and receptor coordinates in the ligand frame. */
+att_status att_debug_beads_modes(att_ctx* ctx, att_handle rec, att_handle lig, att_handle recGrid,
+                                 att_handle ligGrid, att_handle params, const att_dof6m* dof,
+                                 float* xyz, float* pot, int* cell, unsigned* nl_pairs,
+                                 float* rec_pot, float* rec_xyz);
+
+/* Rotation matrices R(phi, ssi, rot) of n angle triples as the kernels compute
+ * them (matrixFunctions.h:38-77), for checking them against the host. */
+att_status att_debug_rotmats(att_ctx* ctx, const float* angles /* 3n */, size_t n, float* R /* 9n */);
+
+att_status att_sync(att_ctx* ctx, void* stream);
+
+/* Library build identification (arch, flags). */
+const char* att_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ATTRACT_HIP_H_ */

@@ -62,18 +62,18 @@ int orc_score_rigid(const orc_body* rec, const orc_body* lig, const orc_igrid* i
     SETUP(_f32)
     x.inner = inner; x.outer = outer; x.nl = nl;
     ctx_init_f32(&x);
-    run_f32(&x, dof, n, res, nthreads, 0);
+    run_f32(&x, dof, n, res, 0, nthreads, 0);
     ctx_free_f32(&x);
     return 0;
 }
 
 int orc_score_rigid_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
                         const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
-                        const float* dof, size_t n, double* res, int nthreads) {
+                        const float* dof, size_t n, double* res, double* scale, int nthreads) {
     SETUP(_f64)
     x.inner = inner; x.outer = outer; x.nl = nl;
     ctx_init_f64(&x);
-    run_f64(&x, dof, n, res, nthreads, 0);
+    run_f64(&x, dof, n, res, scale, nthreads, 0);
     ctx_free_f64(&x);
     return 0;
 }
@@ -87,7 +87,7 @@ int orc_beads_rigid(const orc_body* rec, const orc_body* lig, const orc_igrid* i
     scratch_f32 s;
     scratch_alloc_f32(&s, lig->n, rec->n, 0);
     float res[7];
-    pose_rigid_f32(&x, &s, dof, res, out);
+    pose_rigid_f32(&x, &s, dof, res, out, 0);
     scratch_free_f32(&s);
     ctx_free_f32(&x);
     return 0;
@@ -101,7 +101,7 @@ int orc_score_modes(const orc_body* rec, const orc_body* lig, const orc_igrid* r
     x.inner = recInner; x.outer = recOuter; x.nl = recNL;
     x.ligInner = ligInner; x.ligOuter = ligOuter;
     ctx_init_f32(&x);
-    run_f32(&x, dof, n, res, nthreads, 1);
+    run_f32(&x, dof, n, res, 0, nthreads, 1);
     ctx_free_f32(&x);
     return 0;
 }
@@ -110,12 +110,12 @@ int orc_score_modes_f64(const orc_body* rec, const orc_body* lig, const orc_igri
                         const orc_igrid* recOuter, const orc_nlgrid* recNL,
                         const orc_igrid* ligInner, const orc_igrid* ligOuter,
                         const orc_params* par, const float* dof, size_t n, double* res,
-                        int nthreads) {
+                        double* scale, int nthreads) {
     SETUP(_f64)
     x.inner = recInner; x.outer = recOuter; x.nl = recNL;
     x.ligInner = ligInner; x.ligOuter = ligOuter;
     ctx_init_f64(&x);
-    run_f64(&x, dof, n, res, nthreads, 1);
+    run_f64(&x, dof, n, res, scale, nthreads, 1);
     ctx_free_f64(&x);
     return 0;
 }
@@ -131,7 +131,7 @@ int orc_beads_modes(const orc_body* rec, const orc_body* lig, const orc_igrid* r
     scratch_f32 s;
     scratch_alloc_f32(&s, lig->n, rec->n, 1);
     float res[27];
-    pose_modes_f32(&x, &s, dof, res, out);
+    pose_modes_f32(&x, &s, dof, res, out, 0);
     scratch_free_f32(&s);
     ctx_free_f32(&x);
     return 0;

@@ -78,13 +78,15 @@ void orc_ratio_lut(float plateau, float* out);
 void orc_euler2rotmat(float phi, float ssi, float rot, float R[9]);
 void orc_euler2torquemat(float phi, float ssi, float rot, float tm[27]);
 
-/* Rigid scoring: dof = n*6 {pos, ang}, res = n*7 {E, pos, ang} (Result_6D). */
+/* Rigid scoring: dof = n*6 {pos, ang}, res = n*7 {E, pos, ang} (Result_6D).
+ * The _f64 variants optionally return `scale` (same shape as res): the sum of
+ * |terms| behind each output component, the denominator of the parity bound. */
 int orc_score_rigid(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
                     const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
                     const float* dof, size_t n, float* res, int nthreads);
 int orc_score_rigid_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
                         const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
-                        const float* dof, size_t n, double* res, int nthreads);
+                        const float* dof, size_t n, double* res, double* scale, int nthreads);
 int orc_beads_rigid(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
                     const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
                     const float* dof, orc_beads* out);
@@ -100,7 +102,7 @@ int orc_score_modes_f64(const orc_body* rec, const orc_body* lig, const orc_igri
                         const orc_igrid* recOuter, const orc_nlgrid* recNL,
                         const orc_igrid* ligInner, const orc_igrid* ligOuter,
                         const orc_params* par, const float* dof, size_t n, double* res,
-                        int nthreads);
+                        double* scale, int nthreads);
 int orc_beads_modes(const orc_body* rec, const orc_body* lig, const orc_igrid* recInner,
                     const orc_igrid* recOuter, const orc_nlgrid* recNL,
                     const orc_igrid* ligInner, const orc_igrid* ligOuter, const orc_params* par,

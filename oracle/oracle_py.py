@@ -87,22 +87,23 @@ def oracle_lib():
         if not os.path.exists(path):
             raise RuntimeError(f"{path} missing: build the oracle (make -C oracle)")
         L = C.CDLL(path)
-        for name in ("orc_score_rigid", "orc_score_rigid_f64"):
-            f = getattr(L, name)
-            f.restype = C.c_int
-            f.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
-                          C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(OrcParamsC), P,
-                          C.c_size_t, P, C.c_int]
+        rig = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC), C.POINTER(IGridC),
+               C.POINTER(NLGridC), C.POINTER(OrcParamsC), P, C.c_size_t, P]
+        L.orc_score_rigid.restype = C.c_int
+        L.orc_score_rigid.argtypes = rig + [C.c_int]
+        L.orc_score_rigid_f64.restype = C.c_int
+        L.orc_score_rigid_f64.argtypes = rig + [P, C.c_int]
         L.orc_beads_rigid.restype = C.c_int
         L.orc_beads_rigid.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(NLGridC),
                                       C.POINTER(OrcParamsC), P, C.POINTER(BeadsC)]
-        for name in ("orc_score_modes", "orc_score_modes_f64"):
-            f = getattr(L, name)
-            f.restype = C.c_int
-            f.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
-                          C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
-                          C.POINTER(IGridC), C.POINTER(OrcParamsC), P, C.c_size_t, P, C.c_int]
+        mod = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC), C.POINTER(IGridC),
+               C.POINTER(NLGridC), C.POINTER(IGridC), C.POINTER(IGridC), C.POINTER(OrcParamsC),
+               P, C.c_size_t, P]
+        L.orc_score_modes.restype = C.c_int
+        L.orc_score_modes.argtypes = mod + [C.c_int]
+        L.orc_score_modes_f64.restype = C.c_int
+        L.orc_score_modes_f64.argtypes = mod + [P, C.c_int]
         L.orc_beads_modes.restype = C.c_int
         L.orc_beads_modes.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
@@ -156,26 +157,40 @@ def ref_lib():
 # ---------------------------------------------------------------- oracle API
 
 def score_rigid(wl, dofs=None, nthreads=1, f64=False):
+    """FP32 results (n, 7); with f64=True the FP64 recompute and its condition
+    scale (sum of |terms| per component), both (n, 7) float64."""
     L = oracle_lib()
     d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
     n = d.shape[0]
-    res = np.zeros((n, 7), np.float64 if f64 else np.float32)
-    f = L.orc_score_rigid_f64 if f64 else L.orc_score_rigid
-    f(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)), C.byref(igrid_c(wl.rec_inner)),
-      C.byref(igrid_c(wl.rec_outer)), C.byref(nl_c(wl.rec_nl)), C.byref(params_c(wl.params)),
-      _p(d), n, _p(res), nthreads)
+    args = (C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)), C.byref(igrid_c(wl.rec_inner)),
+            C.byref(igrid_c(wl.rec_outer)), C.byref(nl_c(wl.rec_nl)),
+            C.byref(params_c(wl.params)), _p(d), n)
+    if f64:
+        res = np.zeros((n, 7), np.float64)
+        scale = np.zeros((n, 7), np.float64)
+        L.orc_score_rigid_f64(*args, _p(res), _p(scale), nthreads)
+        return res, scale
+    res = np.zeros((n, 7), np.float32)
+    L.orc_score_rigid(*args, _p(res), nthreads)
     return res
 
 
 def score_modes(wl, dofs=None, nthreads=1, f64=False):
+    """FP32 results (n, 27); with f64=True the FP64 recompute and its scale."""
     L = oracle_lib()
     d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
     n = d.shape[0]
-    res = np.zeros((n, 27), np.float64 if f64 else np.float32)
-    f = L.orc_score_modes_f64 if f64 else L.orc_score_modes
-    f(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)), C.byref(igrid_c(wl.rec_inner)),
-      C.byref(igrid_c(wl.rec_outer)), C.byref(nl_c(wl.rec_nl)), C.byref(igrid_c(wl.lig_inner)),
-      C.byref(igrid_c(wl.lig_outer)), C.byref(params_c(wl.params)), _p(d), n, _p(res), nthreads)
+    args = (C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)), C.byref(igrid_c(wl.rec_inner)),
+            C.byref(igrid_c(wl.rec_outer)), C.byref(nl_c(wl.rec_nl)),
+            C.byref(igrid_c(wl.lig_inner)), C.byref(igrid_c(wl.lig_outer)),
+            C.byref(params_c(wl.params)), _p(d), n)
+    if f64:
+        res = np.zeros((n, 27), np.float64)
+        scale = np.zeros((n, 27), np.float64)
+        L.orc_score_modes_f64(*args, _p(res), _p(scale), nthreads)
+        return res, scale
+    res = np.zeros((n, 27), np.float32)
+    L.orc_score_modes(*args, _p(res), nthreads)
     return res
 
 

@@ -1,0 +1,176 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (needs a GPU).
+
+Bit-exact: rotation matrices, transformed coordinates, per-bead forces and
+energies, grid cell indices, neighbour-list cells and pair counts.
+Tolerance (tests/parity.py): per-pose sums within 1e-5 of the condition scale.
+"""
+import numpy as np
+import pytest
+
+from parity import TOL, cond_err, forcelim_safe, summarize
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gpuattract_amd import hip
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+def test_rotmats_bit_exact(ctx, O):
+    rng = np.random.default_rng(5)
+    ang = np.concatenate([
+        rng.uniform(-7, 7, (200000, 3)),
+        rng.uniform(-1e-3, 1e-3, (1000, 3)),
+        rng.uniform(-100, 100, (1000, 3)),
+        np.array([[0, 0, 0], [np.pi, 0, 0], [0, np.pi / 2, 0], [-0.0, 0.0, -0.0]]),
+    ]).astype(np.float32)
+    R = ctx.rotmats(ang)
+    ref = np.stack([O.rotmat(*a) for a in ang[::97]])
+    assert np.array_equal(R[::97], ref)
+    # all angle triples: against the oracle's formula evaluated in numpy
+    c = lambda v: np.cos(v.astype(np.float64)).astype(np.float32)
+    s = lambda v: np.sin(v.astype(np.float64)).astype(np.float32)
+    phi, ssi, rot = ang[:, 0], ang[:, 1], ang[:, 2]
+    cS, cP, sS, sP, cR, sR = c(ssi), c(phi), s(ssi), s(phi), c(rot), s(rot)
+    assert np.array_equal(R[:, 8], cS)
+    assert np.array_equal(R[:, 2], sS * cP)
+    assert np.array_equal(R[:, 0], cR * (cS * cP) + sR * sP)
+    assert np.array_equal(R[:, 7], -sR * sS)
+
+
+def _check_beads(ctx, hs, wl, O, dof):
+    g = ctx.beads(hs, wl, dof)
+    o = O.beads(wl, dof)
+    assert np.array_equal(g["xyz"], o["xyz"])
+    assert np.array_equal(g["cell"], o["cell"])
+    assert np.array_equal(g["nl_pairs"], o["nl_pairs"])
+    assert np.array_equal(g["pot"], o["pot"])
+    if wl.modes:
+        assert np.array_equal(g["rec_pot"], o["rec_pot"])
+        assert np.array_equal(g["rec_xyz"], o["rec_xyz"])
+    return o
+
+
+def test_beads_rigid_bit_exact(ctx, wl_rigid, O):
+    hs = ctx.upload_workload(wl_rigid)
+    used_nl = 0
+    for k in range(0, 300, 7):
+        o = _check_beads(ctx, hs, wl_rigid, O, wl_rigid.dofs[k])
+        used_nl += int((o["nl_pairs"] > 0).sum())
+    assert used_nl > 0  # the neighbour-list term was exercised
+
+
+def test_beads_modes_bit_exact(ctx, wl_modes, O):
+    hs = ctx.upload_workload(wl_modes)
+    for k in range(0, 200, 9):
+        _check_beads(ctx, hs, wl_modes, O, wl_modes.dofs[k])
+
+
+def _rigid_parity(ctx, wl, O, nthreads=8):
+    hs = ctx.upload_workload(wl)
+    g = ctx.score_rigid(hs, wl.dofs)
+    o32 = O.score_rigid(wl, nthreads=nthreads)
+    o64, sc = O.score_rigid(wl, nthreads=nthreads, f64=True)
+    ok = forcelim_safe(o64)
+    assert ok.mean() > 0.99
+    e32 = cond_err(g[ok], o32[ok], sc[ok])
+    e64 = cond_err(g[ok], o64[ok], sc[ok])
+    print(summarize("rigid gpu vs oracle fp32", g[ok], o32[ok], sc[ok]))
+    print(summarize("rigid gpu vs fp64", g[ok], o64[ok], sc[ok]))
+    assert e32.max() <= TOL, e32.max(axis=0)
+    assert e64.max() <= TOL, e64.max(axis=0)
+    return g, o32
+
+
+def test_score_rigid_small(ctx, wl_rigid, O):
+    _rigid_parity(ctx, wl_rigid, O)
+
+
+def test_score_rigid_1000_300(ctx, W, O):
+    wl = W.config_rigid_1000_300(10000)
+    g, _ = _rigid_parity(ctx, wl, O, nthreads=16)
+    assert np.all(np.isfinite(g))
+
+
+def test_score_modes_small(ctx, wl_modes, O):
+    hs = ctx.upload_workload(wl_modes)
+    g = ctx.score_modes(hs, wl_modes.dofs)
+    o32 = O.score_modes(wl_modes, nthreads=8)
+    o64, sc = O.score_modes(wl_modes, nthreads=8, f64=True)
+    ok = forcelim_safe(o64)
+    cols = [c for c in range(27) if sc[:, c].max() > 0]
+    e = cond_err(g[ok][:, cols], o32[ok][:, cols], sc[ok][:, cols])
+    print(summarize("modes gpu vs oracle fp32", g[ok][:, cols], o32[ok][:, cols], sc[ok][:, cols]))
+    assert e.max() <= TOL, e.max(axis=0)
+    # unused mode slots are exactly zero
+    assert np.all(g[:, 7 + wl_modes.rec.M:17] == 0) and np.all(g[:, 17 + wl_modes.lig.M:] == 0)
+
+
+def test_device_resident_matches_host_path(ctx, wl_modes, wl_rigid):
+    torch = pytest.importorskip("torch")
+    dev = torch.device("cuda:0")
+    for wl, width, out_w in ((wl_rigid, 6, 7), (wl_modes, 26, 27)):
+        hs = ctx.upload_workload(wl)
+        host = ctx.score_modes(hs, wl.dofs) if wl.modes else ctx.score_rigid(hs, wl.dofs)
+        d_in = torch.from_numpy(np.ascontiguousarray(wl.dofs)).to(dev)
+        d_out = torch.zeros((wl.dofs.shape[0], out_w), dtype=torch.float32, device=dev)
+        stream = torch.cuda.current_stream().cuda_stream
+        f = ctx.score_modes_dev if wl.modes else ctx.score_rigid_dev
+        f(hs, d_in.data_ptr(), wl.dofs.shape[0], d_out.data_ptr(), stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out.cpu().numpy(), host)
+
+
+def test_split_and_permutation_invariance(ctx, wl_rigid):
+    hs = ctx.upload_workload(wl_rigid)
+    full = ctx.score_rigid(hs, wl_rigid.dofs)
+    parts = np.concatenate([ctx.score_rigid(hs, wl_rigid.dofs[a:b])
+                            for a, b in ((0, 1), (1, 77), (77, 300))])
+    assert np.array_equal(full, parts)
+    perm = np.random.default_rng(1).permutation(300)
+    assert np.array_equal(ctx.score_rigid(hs, wl_rigid.dofs[perm]), full[perm])
+
+
+def test_edge_cases(ctx, W, O):
+    from gpuattract_amd import hip
+    # type-0 beads, 12-6 potential, constant dielectric, one-bead ligand
+    for kw in (dict(type0_frac=0.2), dict(shape=12), dict(dielec=0), dict(charged_frac=1.0)):
+        wl = W.make_workload("edge", 21, 150, 40, 64, **kw)
+        hs = ctx.upload_workload(wl)
+        _check_beads(ctx, hs, wl, O, wl.dofs[3])
+        g = ctx.score_rigid(hs, wl.dofs)
+        o64, sc = O.score_rigid(wl, f64=True)
+        ok = forcelim_safe(o64)
+        assert cond_err(g[ok], o64[ok], sc[ok]).max() <= TOL
+    wl = W.make_workload("one bead", 22, 100, 1, 32)
+    hs = ctx.upload_workload(wl)
+    g = ctx.score_rigid(hs, wl.dofs)
+    o64, sc = O.score_rigid(wl, f64=True)
+    assert cond_err(g, o64, sc).max() <= TOL
+    # poses far outside every grid score exactly zero
+    far = wl.dofs.copy()
+    far[:, :3] = 1e4
+    assert np.all(ctx.score_rigid(hs, far) == 0)
+    # NaN poses do not fault and stay outside the grids
+    nan = wl.dofs[:4].copy()
+    nan[:, 0] = np.nan
+    ctx.score_rigid(hs, nan)
+    # empty request
+    assert ctx.score_rigid(hs, wl.dofs[:0]).shape == (0, 7)
+    # error paths
+    with pytest.raises(hip.AttError) as e:
+        ctx.score_rigid(dict(hs, rec=987654), wl.dofs)
+    assert e.value.status == 4
+    with pytest.raises(hip.AttError) as e:
+        ctx.score_modes(dict(hs, lig_grid=hs["params"]), np.zeros((2, 26), np.float32))
+    assert e.value.status == 4
+    # a ligand type beyond the parameter table is rejected before any launch
+    small = W.make_params(3, T=4)
+    hp = ctx.upload_params(small)
+    with pytest.raises(hip.AttError) as e:
+        ctx.score_rigid(dict(hs, params=hp), wl.dofs)
+    assert e.value.status == 1
