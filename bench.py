@@ -79,7 +79,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
     _package()
     from gpuattract_amd import hip, traffic, workload as W
 
@@ -88,33 +87,31 @@ def main():
     log(f"workload built in {time.perf_counter() - t0:.1f} s: {wl.name}, {wl.dofs.shape[0]} poses")
     ctx = hip.Context(0)
     hs = ctx.upload_workload(wl)
-    dev = torch.device("cuda:0")
     n = wl.dofs.shape[0]
     out_w = 27 if wl.modes else 7
-    d_in = torch.from_numpy(np.ascontiguousarray(wl.dofs)).to(dev)
-    d_out = torch.zeros((n, out_w), dtype=torch.float32, device=dev)
-    stream = torch.cuda.current_stream()
-    sh = stream.cuda_stream
+    d_in = ctx.to_device(wl.dofs)
+    d_out = ctx.dev_alloc(n * out_w * 4)
+    sh = ctx.stream()  # the stream every launch below is issued on
     score = ctx.score_modes_dev if wl.modes else ctx.score_rigid_dev
 
     def step():
-        score(hs, d_in.data_ptr(), n, d_out.data_ptr(), sh)
+        score(hs, d_in.ptr, n, d_out.ptr, sh)
 
     for _ in range(args.warmup):
         step()
-    torch.cuda.synchronize()
+    ctx.sync(sh)
     log(f"warm-up done ({args.warmup} steps)")
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
+    starts = [ctx.event() for _ in range(args.steps)]
+    ends = [ctx.event() for _ in range(args.steps)]
+    ctx.sync(sh)
     t0 = time.perf_counter()
     for k in range(args.steps):
-        starts[k].record(stream)
+        ctx.record(starts[k], sh)
         step()
-        ends[k].record(stream)
-    torch.cuda.synchronize()
+        ctx.record(ends[k], sh)
+    ctx.sync(sh)
     wall = time.perf_counter() - t0
-    launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    launch_ms = [ctx.elapsed_ms(s, e) for s, e in zip(starts, ends)]
     avg_launch = float(np.mean(launch_ms)) / 1e3
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {wall:.3f} s, {value:.4g} poses/s, launch avg {avg_launch*1e3:.3f} ms")
@@ -145,7 +142,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu, ref = cpu_baseline(wl, args)
-        gpu = d_out[:ref.shape[0]].cpu().numpy()
+        gpu = ctx.to_host(d_out, (n, out_w))[:ref.shape[0]]
         rel = np.abs(gpu - ref) / np.maximum(np.abs(ref), 1e-30)
         cpu["gpu_vs_cpu_median_rel_err"] = float(np.median(rel[:, 0]))
         log(f"cpu baseline: {cpu['value']:.4g} poses/s on {cpu['cores']} threads")

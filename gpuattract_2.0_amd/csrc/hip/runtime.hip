@@ -584,4 +584,85 @@ att_status att_sync(att_ctx* c, void* stream) {
     return ATT_OK;
 }
 
+att_status att_dev_alloc(att_ctx* c, size_t bytes, void** d_ptr) {
+    if (!c || !d_ptr) return fail(c, ATT_ERR_INVALID, "null argument");
+    *d_ptr = nullptr;
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipMalloc(d_ptr, bytes ? bytes : 1));
+    return ATT_OK;
+}
+
+att_status att_dev_free(att_ctx* c, void* d_ptr) {
+    if (!c) return ATT_ERR_INVALID;
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipFree(d_ptr));
+    return ATT_OK;
+}
+
+att_status att_host_alloc(att_ctx* c, size_t bytes, void** h_ptr) {
+    if (!c || !h_ptr) return fail(c, ATT_ERR_INVALID, "null argument");
+    *h_ptr = nullptr;
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return ATT_OK;
+}
+
+att_status att_host_free(att_ctx* c, void* h_ptr) {
+    if (!c) return ATT_ERR_INVALID;
+    ATT_HIP(c, hipHostFree(h_ptr));
+    return ATT_OK;
+}
+
+att_status att_memcpy_h2d(att_ctx* c, void* d_dst, const void* src, size_t bytes, void* stream) {
+    if (!c || (bytes && (!d_dst || !src))) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, s));
+    ATT_HIP(c, hipStreamSynchronize(s));
+    return ATT_OK;
+}
+
+att_status att_memcpy_d2h(att_ctx* c, void* dst, const void* d_src, size_t bytes, void* stream) {
+    if (!c || (bytes && (!dst || !d_src))) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, s));
+    ATT_HIP(c, hipStreamSynchronize(s));
+    return ATT_OK;
+}
+
+att_status att_get_stream(att_ctx* c, void** stream) {
+    if (!c || !stream) return fail(c, ATT_ERR_INVALID, "null argument");
+    *stream = (void*)c->stream;
+    return ATT_OK;
+}
+
+att_status att_event_create(att_ctx* c, void** ev) {
+    if (!c || !ev) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipEvent_t e;
+    ATT_HIP(c, hipEventCreate(&e));
+    *ev = (void*)e;
+    return ATT_OK;
+}
+
+att_status att_event_destroy(att_ctx* c, void* ev) {
+    if (!c) return ATT_ERR_INVALID;
+    ATT_HIP(c, hipEventDestroy((hipEvent_t)ev));
+    return ATT_OK;
+}
+
+att_status att_event_record(att_ctx* c, void* ev, void* stream) {
+    if (!c || !ev) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipEventRecord((hipEvent_t)ev, stream ? (hipStream_t)stream : c->stream));
+    return ATT_OK;
+}
+
+att_status att_event_elapsed_ms(att_ctx* c, void* a, void* b, float* ms) {
+    if (!c || !a || !b || !ms) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipEventSynchronize((hipEvent_t)b));
+    ATT_HIP(c, hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+    return ATT_OK;
+}
+
 }  // extern "C"

@@ -1,9 +1,10 @@
 """ctypes binding of libattract_hip.so (include/attract_hip.h).
 
 Host-buffer calls take numpy arrays; the *_dev calls take device pointers
-(e.g. torch tensors on cuda via .data_ptr()) and a hipStream_t (torch's
-stream handle).  Loading fails loudly when the library is missing: there is no
-CPU fallback.
+from the library's own allocator (Context.dev_alloc / to_device) and a
+hipStream_t (None = the context's stream), so the process needs no other GPU
+runtime.  Loading fails loudly when the library is missing: there is no CPU
+fallback.
 """
 import ctypes as C
 import os
@@ -77,12 +78,41 @@ def lib():
         L.att_debug_rotmats.argtypes = [P, P, C.c_size_t, P]
         L.att_sync.argtypes = [P, P]
         L.att_build_info.restype = C.c_char_p
+        L.att_dev_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
+        L.att_dev_free.argtypes = [P, P]
+        L.att_host_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
+        L.att_host_free.argtypes = [P, P]
+        L.att_memcpy_h2d.argtypes = [P, P, P, C.c_size_t, P]
+        L.att_memcpy_d2h.argtypes = [P, P, P, C.c_size_t, P]
+        L.att_get_stream.argtypes = [P, C.POINTER(P)]
+        L.att_event_create.argtypes = [P, C.POINTER(P)]
+        L.att_event_destroy.argtypes = [P, P]
+        L.att_event_record.argtypes = [P, P, P]
+        L.att_event_elapsed_ms.argtypes = [P, P, P, C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(P)
+
+
+class DeviceBuffer:
+    """Device allocation owned by a Context (freed on release or GC)."""
+
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def release(self):
+        if self.ptr and self.ctx.h:
+            self.ctx.L.att_dev_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
 
 
 class Context:
@@ -185,6 +215,41 @@ class Context:
 
     def sync(self, stream=None):
         self._chk(self.L.att_sync(self.h, stream))
+
+    # -- device buffers and timing ----------------------------------------
+    def dev_alloc(self, nbytes):
+        p = P()
+        self._chk(self.L.att_dev_alloc(self.h, nbytes, C.byref(p)))
+        return DeviceBuffer(self, p.value, nbytes)
+
+    def to_device(self, arr):
+        a = np.ascontiguousarray(arr)
+        buf = self.dev_alloc(a.nbytes)
+        self._chk(self.L.att_memcpy_h2d(self.h, buf.ptr, _p(a), a.nbytes, None))
+        return buf
+
+    def to_host(self, buf, shape, dtype=np.float32):
+        out = np.empty(shape, dtype)
+        self._chk(self.L.att_memcpy_d2h(self.h, _p(out), buf.ptr, out.nbytes, None))
+        return out
+
+    def stream(self):
+        s = P()
+        self._chk(self.L.att_get_stream(self.h, C.byref(s)))
+        return s.value
+
+    def event(self):
+        e = P()
+        self._chk(self.L.att_event_create(self.h, C.byref(e)))
+        return e.value
+
+    def record(self, ev, stream=None):
+        self._chk(self.L.att_event_record(self.h, ev, stream))
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        self._chk(self.L.att_event_elapsed_ms(self.h, a, b, C.byref(ms)))
+        return ms.value
 
     # -- diagnostics -----------------------------------------------------
     def beads(self, hs, wl, dof):

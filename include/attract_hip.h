@@ -156,6 +156,23 @@ att_status att_debug_rotmats(att_ctx* ctx, const float* angles /* 3n */, size_t 
 
 att_status att_sync(att_ctx* ctx, void* stream);
 
+/* --- device buffers, copies and timing -------------------------------------
+ * Replace the reference's DeviceAllocator / HostPinnedAllocator
+ * (allocator/DeviceAllocator.tpp:19-30, allocator/HostPinnedAllocator.tpp:19-30)
+ * and the CUDA events of GPUEnergyService6D::Private (:101-114), so that a
+ * caller needs no other GPU runtime in its process. */
+att_status att_dev_alloc(att_ctx* ctx, size_t bytes, void** d_ptr);
+att_status att_dev_free(att_ctx* ctx, void* d_ptr);
+att_status att_host_alloc(att_ctx* ctx, size_t bytes, void** h_ptr); /* pinned */
+att_status att_host_free(att_ctx* ctx, void* h_ptr);
+att_status att_memcpy_h2d(att_ctx* ctx, void* d_dst, const void* src, size_t bytes, void* stream);
+att_status att_memcpy_d2h(att_ctx* ctx, void* dst, const void* d_src, size_t bytes, void* stream);
+att_status att_get_stream(att_ctx* ctx, void** stream);
+att_status att_event_create(att_ctx* ctx, void** ev);
+att_status att_event_destroy(att_ctx* ctx, void* ev);
+att_status att_event_record(att_ctx* ctx, void* ev, void* stream);
+att_status att_event_elapsed_ms(att_ctx* ctx, void* ev_start, void* ev_stop, float* ms);
+
 /* Library build identification (arch, flags). */
 const char* att_build_info(void);
 
