@@ -73,6 +73,9 @@ def main():
     ap.add_argument("--cpu-poses", type=int, default=20000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-poses", type=int, default=5000)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="also run the plain copy kernel over this many bytes "
+                         "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
@@ -115,6 +118,15 @@ def main():
     avg_launch = float(np.mean(launch_ms)) / 1e3
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {wall:.3f} s, {value:.4g} poses/s, launch avg {avg_launch*1e3:.3f} ms")
+
+    if args.copy_calibration:
+        nb = args.copy_calibration // 16 * 16
+        a, bb = ctx.dev_alloc(nb), ctx.dev_alloc(nb)
+        ctx.copy(bb, a, nb, sh)
+        ctx.sync(sh)
+        log(f"copy calibration: {nb} bytes read + {nb} bytes written")
+        a.release()
+        bb.release()
 
     bpp, bpp_sd = traffic.bytes_per_pose(wl)
     achieved = bpp * n / avg_launch

@@ -17,9 +17,12 @@ constexpr int kWave = 64;
 constexpr unsigned kMaxTypesLDS = 32;  // pair table staged in LDS up to 32x32 types
 
 // Interpolation grid (model/IntrplGrid.h:34-209) with the constants its
-// constructor derives (model/IntrplGrid.tpp:30-41).
+// constructor derives (model/IntrplGrid.tpp:30-41).  Stored as voxel octets:
+// for every (grid, iz, iy, ix) with i < n-1 the eight corner float4 values, in
+// the order trilinearInterpolation sums them (000, 100, 010, 001, 101, 011,
+// 110, 111), contiguous and 128-byte aligned — one cache line per lookup.
 struct DGrid {
-    const float4* data;
+    const float4* oct;
     unsigned nx, ny, nz, nxy, nvox, ngrids;
     float lox, loy, loz, hix, hiy, hiz;
     float dVox, dVox_inv, vol_inv;
@@ -41,9 +44,9 @@ struct DProt {
     const unsigned* type;
     const unsigned* mtype;
     const float* modes;   // [c][m][n]: coalesced over beads
-    const float* modesT;  // [n][m][c]: one contiguous run per bead (gathers)
+    const float4* modesT; // per bead one row of 3*M floats [m][c], padded to a float4 multiple
     const float* modeForce;
-    unsigned n, M;
+    unsigned n, M, rowT;  // rowT = float4 per modesT row
 };
 
 struct DPar {
@@ -178,10 +181,10 @@ __device__ __forceinline__ float4 trilin(const DGrid& g, float px, float py, flo
     const float w5 = (bx * ay) * az;
     const float w6 = tmpMin_xy * bz;
     const float w7 = tmpMin_xy * az;
-    const unsigned base = type * g.nvox + g.nx * (iz * g.ny + iy) + ix;
-    const float4* d = g.data + base;
-    const float4 c0 = d[0], c1 = d[1], c2 = d[g.nx], c6 = d[g.nx + 1];
-    const float4 c3 = d[g.nxy], c4 = d[g.nxy + 1], c5 = d[g.nxy + g.nx], c7 = d[g.nxy + g.nx + 1];
+    const size_t o = (((size_t)type * (g.nz - 1) + iz) * (g.ny - 1) + iy) * (g.nx - 1) + ix;
+    const float4* d = g.oct + 8 * o;
+    const float4 c0 = d[0], c1 = d[1], c2 = d[2], c3 = d[3];
+    const float4 c4 = d[4], c5 = d[5], c6 = d[6], c7 = d[7];
     float4 V;
     V.x = c0.x * w0; V.y = c0.y * w0; V.z = c0.z * w0; V.w = c0.w * w0;
     V.x = V.x + c1.x * w1; V.y = V.y + c1.y * w1; V.z = V.z + c1.z * w1; V.w = V.w + c1.w * w1;
@@ -268,16 +271,21 @@ __device__ __forceinline__ void coulomb(float dr2_inv, float dx, float dy, float
 }
 
 // Neighbour-list term of one ligand bead (model/neighborlist.h:47-158 and the
-// receptor reaction of model/neighborlist_modes.h:68-194).  RecPos(j) returns
-// the receptor position used for the distance; Reaction(j, fx, fy, fz) receives
-// the reaction force of pair j (no-op for rigid scoring).  Returns false when
-// the bead is outside the neighbour-list grid.
-template <class RecPos, class Reaction>
-__device__ __forceinline__ bool nl_bead(const DNL& nl, const DProt& rec, const DPar& par,
+// receptor reaction of model/neighborlist_modes.h:68-194).  Accessors:
+//   recPos(j)    -> float3 receptor position used for the distance
+//   recType(j)   -> receptor type
+//   recCharge(j) -> receptor charge (only read when the ligand bead is charged)
+//   reaction(j, fx, fy, fz) receives the pair's reaction force (no-op for rigid)
+// List entries are fetched four at a time ahead of use; the pairs are still
+// accumulated one after another in list order, as the reference does.
+// Returns false when the bead is outside the neighbour-list grid.
+template <class RecPos, class RecType, class RecCharge, class Reaction>
+__device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
                                         const att_pair_params* tab, RecPos recPos,
-                                        Reaction reaction, float px, float py, float pz,
-                                        unsigned ligType, float ligCharge, float4& acc,
-                                        int* cell, unsigned* npairs) {
+                                        RecType recType, RecCharge recCharge, Reaction reaction,
+                                        float px, float py, float pz, unsigned ligType,
+                                        float ligCharge, float4& acc, int* cell,
+                                        unsigned* npairs) {
     acc = make_float4(0.f, 0.f, 0.f, 0.f);
     if (ligType == 0) return false;
     if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
@@ -292,43 +300,55 @@ __device__ __forceinline__ bool nl_bead(const DNL& nl, const DProt& rec, const D
     float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f;
     unsigned pairs = 0;
     const unsigned lt1 = ligType - 1;
-    for (unsigned j = 0; j < d.x; ++j) {
-        const unsigned nIdx = nl.list[d.y + j];
-        const float4 rp = recPos(nIdx);
-        float dx = px - rp.x;
-        float dy = py - rp.y;
-        float dz = pz - rp.z;
-        const float dr2 = dx * dx + dy * dy + dz * dz;
-        if (!(dr2 <= nl.P2)) continue;  // outOfPlateau (model/NLGrid.h:133-135); NaN skipped
-        ++pairs;
-        const float dr2_inv = (float)(1.0 / (double)dr2);
-        dx *= dr2_inv; dy *= dr2_inv; dz *= dr2_inv;
-        const unsigned rt = rec.type[nIdx];
-        // A listed bead without a valid type would index outside the table
-        // (the reference reads out of bounds there); such pairs are skipped.
-        if (rt - 1u >= par.T) continue;
-        const att_pair_params p = tab[par.T * (rt - 1) + lt1];
-        float vx, vy, vz, ve;
-        float rfx = 0.f, rfy = 0.f, rfz = 0.f;
-        lj(dr2, dr2_inv, dx, dy, dz, p, par.shape, vx, vy, vz, ve);
-        fx += vx; fy += vy; fz += vz; e += ve;
-        rfx -= vx; rfy -= vy; rfz -= vz;
-        const float qlr = ligCharge * rp.w * par.ffelec;
-        const bool calc_elec = fabsf(qlr) > 0.001f;
-        const float ratio = nl.ratio[(int)(dr2 * 10000.f)];
-        const float rdx = ratio * dx, rdy = ratio * dy, rdz = ratio * dz;
-        lj(nl.P2, nl.P2_inv, rdx, rdy, rdz, p, par.shape, vx, vy, vz, ve);
-        fx -= vx; fy -= vy; fz -= vz; e -= ve;
-        rfx += vx; rfy += vy; rfz += vz;
-        if (calc_elec) {
-            coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+    const bool ligCharged = ligCharge != 0.f;  // else q_lig*q_rec*ffelec = 0: no Coulomb term
+    const uint32_t* lst = nl.list + d.y;
+    for (unsigned j0 = 0; j0 < d.x; j0 += 4) {
+        unsigned idx[4];
+        float3 rp[4];
+        unsigned rt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) idx[k] = lst[j0 + k < d.x ? j0 + k : j0];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { rp[k] = recPos(idx[k]); rt[k] = recType(idx[k]); }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (j0 + k >= d.x) break;
+            const unsigned nIdx = idx[k];
+            float dx = px - rp[k].x;
+            float dy = py - rp[k].y;
+            float dz = pz - rp[k].z;
+            const float dr2 = dx * dx + dy * dy + dz * dz;
+            if (!(dr2 <= nl.P2)) continue;  // outOfPlateau (model/NLGrid.h:133-135); NaN skipped
+            // A listed bead without a valid type would index outside the table
+            // (the reference reads out of bounds there); such pairs are skipped.
+            if (rt[k] - 1u >= par.T) continue;
+            ++pairs;
+            const float dr2_inv = (float)(1.0 / (double)dr2);
+            dx *= dr2_inv; dy *= dr2_inv; dz *= dr2_inv;
+            const att_pair_params p = tab[par.T * (rt[k] - 1) + lt1];
+            float vx, vy, vz, ve;
+            float rfx = 0.f, rfy = 0.f, rfz = 0.f;
+            lj(dr2, dr2_inv, dx, dy, dz, p, par.shape, vx, vy, vz, ve);
             fx += vx; fy += vy; fz += vz; e += ve;
             rfx -= vx; rfy -= vy; rfz -= vz;
-            coulomb(nl.P2_inv, rdx, rdy, rdz, qlr, par.dielec, vx, vy, vz, ve);
+            const float ratio = nl.ratio[(int)(dr2 * 10000.f)];
+            const float rdx = ratio * dx, rdy = ratio * dy, rdz = ratio * dz;
+            lj(nl.P2, nl.P2_inv, rdx, rdy, rdz, p, par.shape, vx, vy, vz, ve);
             fx -= vx; fy -= vy; fz -= vz; e -= ve;
             rfx += vx; rfy += vy; rfz += vz;
+            if (ligCharged) {
+                const float qlr = ligCharge * recCharge(nIdx) * par.ffelec;
+                if (fabsf(qlr) > 0.001f) {
+                    coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+                    fx += vx; fy += vy; fz += vz; e += ve;
+                    rfx -= vx; rfy -= vy; rfz -= vz;
+                    coulomb(nl.P2_inv, rdx, rdy, rdz, qlr, par.dielec, vx, vy, vz, ve);
+                    fx -= vx; fy -= vy; fz -= vz; e -= ve;
+                    rfx += vx; rfy += vy; rfz += vz;
+                }
+            }
+            reaction(nIdx, rfx, rfy, rfz);
         }
-        reaction(nIdx, rfx, rfy, rfz);
     }
     acc = make_float4(fx, fy, fz, e);
     if (npairs) *npairs = pairs;

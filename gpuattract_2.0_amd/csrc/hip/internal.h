@@ -31,4 +31,9 @@ hipError_t launch_debug_modes(const DProt& rec, const DProt& lig, const DGrid& r
 
 hipError_t launch_debug_rotmats(const float* ang, unsigned n, float* R, hipStream_t s);
 
+hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s);
+
+hipError_t launch_octets(const float4* grid, float4* oct, unsigned nx, unsigned ny, unsigned nz,
+                         unsigned ngrids, hipStream_t s);
+
 }  // namespace att

@@ -114,9 +114,9 @@ att_handle store(att_ctx* c, std::unique_ptr<Obj> o) {
 }
 
 // Grid constants as IntrplGrid's constructor computes them (float arithmetic).
-DGrid make_dgrid(const att_grid_desc& g, const float4* data) {
+DGrid make_dgrid(const att_grid_desc& g, const float4* oct) {
     DGrid d{};
-    d.data = data;
+    d.oct = oct;
     d.nx = g.nx; d.ny = g.ny; d.nz = g.nz;
     d.nxy = g.nx * g.ny;
     d.nvox = g.nx * g.ny * g.nz;
@@ -132,15 +132,39 @@ DGrid make_dgrid(const att_grid_desc& g, const float4* data) {
     return d;
 }
 
+// Upload a [grid][z][y][x] float4 grid and expand it on the device into voxel
+// octets (8x the memory, one 128-byte line per lookup); the compact copy is
+// freed once expanded.
+att_status upload_octets(att_ctx* c, Obj* o, const att_grid_desc& g, float4** out) {
+    const size_t n = (size_t)g.nx * g.ny * g.nz * g.numGrids;
+    const size_t noct = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1) * g.numGrids * 8;
+    void* tmp = nullptr;
+    void* oct = nullptr;
+    ATT_HIP(c, hipMalloc(&oct, sizeof(float4) * noct));
+    o->allocs.push_back(oct);
+    hipError_t e = hipMalloc(&tmp, sizeof(float4) * n);
+    if (e == hipSuccess) e = hipMemcpy(tmp, g.data, sizeof(float4) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = launch_octets(static_cast<const float4*>(tmp), static_cast<float4*>(oct), g.nx, g.ny,
+                          g.nz, g.numGrids, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tmp) (void)hipFree(tmp);
+    if (e != hipSuccess)
+        return fail(c, e == hipErrorOutOfMemory ? ATT_ERR_NOMEM : ATT_ERR_HIP,
+                    std::string("grid upload: ") + hipGetErrorString(e));
+    *out = static_cast<float4*>(oct);
+    return ATT_OK;
+}
+
 att_status check_grid_desc(att_ctx* c, const att_grid_desc* g, const char* what) {
     if (!g || !g->data) return fail(c, ATT_ERR_INVALID, std::string(what) + ": missing grid data");
     if (g->nx < 2 || g->ny < 2 || g->nz < 2 || g->numGrids < 1)
         return fail(c, ATT_ERR_INVALID, std::string(what) + ": grid needs >= 2 points per axis and >= 1 grid");
     if (!(g->spacing > 0.f) || !std::isfinite(g->spacing))
         return fail(c, ATT_ERR_INVALID, std::string(what) + ": grid spacing must be positive");
-    const uint64_t total = (uint64_t)g->nx * g->ny * g->nz * g->numGrids;
-    if (total >= (1ull << 32))
-        return fail(c, ATT_ERR_UNSUPPORTED, std::string(what) + ": grid exceeds 2^32 points");
+    const uint64_t total = (uint64_t)g->nx * g->ny * g->nz * g->numGrids * 8;
+    if (total >= (1ull << 34))
+        return fail(c, ATT_ERR_UNSUPPORTED, std::string(what) + ": grid exceeds 2^31 points");
     return ATT_OK;
 }
 
@@ -229,6 +253,7 @@ att_status att_upload_protein(att_ctx* c, const att_protein_desc* d, att_handle*
     for (unsigned i = 0; i < n; ++i)
         pos[i] = make_float4(d->xyz[i], d->xyz[n + i], d->xyz[2 * (size_t)n + i], d->charge[i]);
     o->maxType = *std::max_element(d->type, d->type + n);
+    if (o->maxType > 0xFFFFu) return fail(c, ATT_ERR_INVALID, "bead type exceeds 65535");
     o->maxMType = *std::max_element(d->mappedType, d->mappedType + n);
     float4* dpos;
     unsigned *dtype, *dmtype;
@@ -243,21 +268,26 @@ att_status att_upload_protein(att_ctx* c, const att_protein_desc* d, att_handle*
     o->d.M = M;
     if (M > 0) {
         const size_t nM = (size_t)n * M;
-        std::vector<float> soa(3 * nM), aos(3 * nM);
+        const unsigned rowT = (3 * M + 3) / 4;  // float4 per padded row
+        std::vector<float> soa(3 * nM), aos((size_t)n * rowT * 4, 0.f);
         for (unsigned cc = 0; cc < 3; ++cc)
             for (unsigned i = 0; i < n; ++i)
                 for (unsigned m = 0; m < M; ++m) {
                     const float v = d->modes[cc * nM + (size_t)i * M + m];
                     soa[(cc * M + m) * (size_t)n + i] = v;
-                    aos[((size_t)i * M + m) * 3 + cc] = v;
+                    aos[(size_t)i * rowT * 4 + 3 * m + cc] = v;
                 }
-        float *dsoa, *daos, *dmf;
+        float *dsoa, *dmf;
+        float4* daos;
         if ((st = dev_copy(c, o.get(), soa.data(), soa.size(), &dsoa)) != ATT_OK) return st;
-        if ((st = dev_copy(c, o.get(), aos.data(), aos.size(), &daos)) != ATT_OK) return st;
+        if ((st = dev_copy(c, o.get(), reinterpret_cast<const float4*>(aos.data()),
+                           (size_t)n * rowT, &daos)) != ATT_OK)
+            return st;
         if ((st = dev_copy(c, o.get(), d->modeForce, M, &dmf)) != ATT_OK) return st;
         o->d.modes = dsoa;
         o->d.modesT = daos;
         o->d.modeForce = dmf;
+        o->d.rowT = rowT;
     }
     *out = store(c, std::move(o));
     return ATT_OK;
@@ -274,10 +304,8 @@ att_status att_upload_grid(att_ctx* c, const att_grid_desc* inner, const att_gri
         return fail(c, ATT_ERR_INVALID, "inner and outer grids differ in the number of types");
     auto o = std::make_unique<GridObj>();
     float4 *din, *dout;
-    const size_t nin = (size_t)inner->nx * inner->ny * inner->nz * inner->numGrids;
-    const size_t nout = (size_t)outer->nx * outer->ny * outer->nz * outer->numGrids;
-    if ((st = dev_copy(c, o.get(), reinterpret_cast<const float4*>(inner->data), nin, &din)) != ATT_OK) return st;
-    if ((st = dev_copy(c, o.get(), reinterpret_cast<const float4*>(outer->data), nout, &dout)) != ATT_OK) return st;
+    if ((st = upload_octets(c, o.get(), *inner, &din)) != ATT_OK) return st;
+    if ((st = upload_octets(c, o.get(), *outer, &dout)) != ATT_OK) return st;
     o->inner = make_dgrid(*inner, din);
     o->outer = make_dgrid(*outer, dout);
     if (nl) {
@@ -581,6 +609,14 @@ att_status att_sync(att_ctx* c, void* stream) {
     if (!c) return ATT_ERR_INVALID;
     ATT_HIP(c, hipSetDevice(c->device));
     ATT_HIP(c, hipStreamSynchronize(stream ? (hipStream_t)stream : c->stream));
+    return ATT_OK;
+}
+
+att_status att_debug_copy(att_ctx* c, void* d_dst, const void* d_src, size_t bytes, void* stream) {
+    if (!c || (bytes && (!d_dst || !d_src))) return fail(c, ATT_ERR_INVALID, "null argument");
+    if (bytes % 16) return fail(c, ATT_ERR_INVALID, "copy size must be a multiple of 16 bytes");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, launch_copy(d_dst, d_src, bytes / 16, stream ? (hipStream_t)stream : c->stream));
     return ATT_OK;
 }
 

@@ -15,6 +15,8 @@
 //           coordinates, folding the receptor reaction straight into the
 //           receptor mode gradients.  Semantics of
 //           service/CPUEnergyService6DModes.tpp:121-327.
+#include <algorithm>
+
 #include "device.h"
 #include "internal.h"
 
@@ -32,6 +34,23 @@ __device__ __forceinline__ const att_pair_params* stage_table(const DPar& par,
 
 struct NoReaction {
     __device__ void operator()(unsigned, float, float, float) const {}
+};
+
+// Receptor accessors reading the protein arrays in global memory.
+struct RecPosG {
+    const float4* pos;
+    __device__ float3 operator()(unsigned j) const {
+        const float4 v = pos[j];
+        return make_float3(v.x, v.y, v.z);
+    }
+};
+struct RecTypeG {
+    const unsigned* type;
+    __device__ unsigned operator()(unsigned j) const { return type[j]; }
+};
+struct RecChargeG {
+    const float4* pos;
+    __device__ float operator()(unsigned j) const { return pos[j].w; }
 };
 
 __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer,
@@ -59,8 +78,8 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
         p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
         float4 pot = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
         float4 a;
-        if (nl_bead(nl, rec, par, tab, [rpos](unsigned j) { return rpos[j]; }, NoReaction{}, p.x,
-                    p.y, p.z, lig.type[i], x.w, a, nullptr, nullptr)) {
+        if (nl_bead(nl, par, tab, RecPosG{rpos}, RecTypeG{rec.type}, RecChargeG{rpos},
+                    NoReaction{}, p.x, p.y, p.z, lig.type[i], x.w, a, nullptr, nullptr)) {
             pot.x = pot.x + a.x; pot.y = pot.y + a.y; pot.z = pot.z + a.z; pot.w = pot.w + a.w;
         }
         acc[0] += pot.w;
@@ -82,22 +101,44 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
     }
 }
 
-// Mode deformation x + sum_m a_m * mode_m (model/transform.h:69-73), modes
-// stored [c][m][n].
-__device__ __forceinline__ V3 deform(const DProt& b, unsigned i, float x, float y, float z,
-                                     const float* amp) {
-    V3 p{x, y, z};
+// Mode vectors of bead i (modes stored [c][m][n], coalesced over beads).
+struct BeadModes {
+    float x[ATT_MODES_MAX], y[ATT_MODES_MAX], z[ATT_MODES_MAX];
+};
+
+__device__ __forceinline__ void load_modes(const DProt& b, unsigned i, BeadModes& md) {
     const size_t nM = (size_t)b.M * b.n;
 #pragma unroll
     for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
-        if (m < b.M) {
+        md.x[m] = m < b.M ? b.modes[(size_t)m * b.n + i] : 0.f;
+        md.y[m] = m < b.M ? b.modes[nM + (size_t)m * b.n + i] : 0.f;
+        md.z[m] = m < b.M ? b.modes[2 * nM + (size_t)m * b.n + i] : 0.f;
+    }
+}
+
+// Mode deformation x + sum_m a_m * mode_m, sequential over modes
+// (model/transform.h:69-73).
+__device__ __forceinline__ V3 deform(const BeadModes& md, unsigned M, float x, float y, float z,
+                                     const float* amp) {
+    V3 p{x, y, z};
+#pragma unroll
+    for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
+        if (m < M) {
             const float a = amp[m];
-            p.x = p.x + a * b.modes[(size_t)m * b.n + i];
-            p.y = p.y + a * b.modes[nM + (size_t)m * b.n + i];
-            p.z = p.z + a * b.modes[2 * nM + (size_t)m * b.n + i];
+            p.x = p.x + a * md.x[m];
+            p.y = p.y + a * md.y[m];
+            p.z = p.z + a * md.z[m];
         }
     }
     return p;
+}
+
+// Accumulates -f . mode_m into g[m] (model/reduction_modes.h:49-55, 83-86).
+__device__ __forceinline__ void mode_dots(const BeadModes& md, unsigned M, float fx, float fy,
+                                          float fz, float* g) {
+#pragma unroll
+    for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
+        if (m < M) g[m] -= fx * md.x[m] + fy * md.y[m] + fz * md.z[m];
 }
 
 constexpr int kModesVals = 13 + 2 * ATT_MODES_MAX;
@@ -113,6 +154,7 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     float* s_rx = reinterpret_cast<float*>(s_dyn + tabSlots);
     float* s_ry = s_rx + rec.n;
     float* s_rz = s_ry + rec.n;
+    unsigned short* s_rt = reinterpret_cast<unsigned short*>(s_rz + rec.n);
     const unsigned pose = blockIdx.x;
     if (pose >= n) return;  // uniform per workgroup
     const int lane = threadIdx.x & (kWave - 1);
@@ -142,45 +184,50 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     for (int m = 0; m < ATT_MODES_MAX; ++m) { mR[m] = 0.f; mL[m] = 0.f; }
 
     // Phase A: receptor beads.
-    const size_t rnM = (size_t)rec.M * rec.n;
     for (unsigned j = threadIdx.x; j < rec.n; j += blockDim.x) {
         const float4 x = rec.pos[j];
-        const V3 df = deform(rec, j, x.x, x.y, x.z, ampR);
+        BeadModes md;
+        load_modes(rec, j, md);
+        const V3 df = deform(md, rec.M, x.x, x.y, x.z, ampR);
         s_rx[j] = df.x; s_ry[j] = df.y; s_rz[j] = df.z;
+        s_rt[j] = (unsigned short)rec.type[j];
         V3 p = matvec(Rinv, df.x, df.y, df.z);
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
         const float4 pot = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
         const V3 f = matvec(R, pot.x, pot.y, pot.z);  // rotate_forces (transform_modes.h:141-161)
-#pragma unroll
-        for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
-            if (m < rec.M)
-                mR[m] -= f.x * rec.modes[(size_t)m * rec.n + j] +
-                         f.y * rec.modes[rnM + (size_t)m * rec.n + j] +
-                         f.z * rec.modes[2 * rnM + (size_t)m * rec.n + j];
+        mode_dots(md, rec.M, f.x, f.y, f.z, mR);
     }
     __syncthreads();
 
     // Phase B: ligand beads.
     const float4* __restrict__ rpos = rec.pos;
-    const float* __restrict__ rmodesT = rec.modesT;
-    const unsigned Mr = rec.M;
-    auto recPos = [=](unsigned j) { return make_float4(s_rx[j], s_ry[j], s_rz[j], rpos[j].w); };
+    const float4* __restrict__ rmodesT = rec.modesT;
+    const unsigned Mr = rec.M, rowT = rec.rowT;
+    auto recPos = [=](unsigned j) { return make_float3(s_rx[j], s_ry[j], s_rz[j]); };
+    auto recType = [=](unsigned j) { return (unsigned)s_rt[j]; };
     auto reaction = [&](unsigned j, float fx, float fy, float fz) {
-        const float* mt = rmodesT + (size_t)j * Mr * 3;
+        const float4* row = rmodesT + (size_t)j * rowT;
+        float mt[4 * ((3 * ATT_MODES_MAX + 3) / 4)];
+#pragma unroll
+        for (unsigned q = 0; q < (3 * ATT_MODES_MAX + 3) / 4; ++q) {
+            const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
+        }
 #pragma unroll
         for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
             if (m < Mr) mR[m] -= fx * mt[3 * m] + fy * mt[3 * m + 1] + fz * mt[3 * m + 2];
     };
-    const size_t lnM = (size_t)lig.M * lig.n;
     for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
         const float4 x = lig.pos[i];
-        const V3 df = deform(lig, i, x.x, x.y, x.z, ampL);
+        BeadModes md;
+        load_modes(lig, i, md);
+        const V3 df = deform(md, lig.M, x.x, x.y, x.z, ampL);
         V3 p = matvec(R, df.x, df.y, df.z);
         p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
         float4 pot = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
         float4 a;
-        if (nl_bead(nl, rec, par, tab, recPos, reaction, p.x, p.y, p.z, lig.type[i], x.w, a,
-                    nullptr, nullptr)) {
+        if (nl_bead(nl, par, tab, recPos, recType, RecChargeG{rpos}, reaction, p.x, p.y, p.z,
+                    lig.type[i], x.w, a, nullptr, nullptr)) {
             pot.x = pot.x + a.x; pot.y = pot.y + a.y; pot.z = pot.z + a.z; pot.w = pot.w + a.w;
         }
         acc[0] += pot.w;
@@ -190,12 +237,7 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         acc[10] += x.x * pot.z; acc[11] += x.y * pot.z; acc[12] += x.z * pot.z;
         // reduceModeForce, ligand (model/reduction_modes.h:28-57): R^T f . mode.
         const V3 fl = matvec(RT, pot.x, pot.y, pot.z);
-#pragma unroll
-        for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
-            if (m < lig.M)
-                mL[m] -= fl.x * lig.modes[(size_t)m * lig.n + i] +
-                         fl.y * lig.modes[lnM + (size_t)m * lig.n + i] +
-                         fl.z * lig.modes[2 * lnM + (size_t)m * lig.n + i];
+        mode_dots(md, lig.M, fl.x, fl.y, fl.z, mL);
     }
 
     // Workgroup reduction: wave butterflies, then the four waves in order.
@@ -256,8 +298,8 @@ __global__ void k_debug_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer, DN
         float4 pp = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, c);
         float4 a;
         unsigned np = 0;
-        if (nl_bead(nl, rec, par, par.tab, [&](unsigned j) { return rec.pos[j]; }, NoReaction{},
-                    p.x, p.y, p.z, lig.type[i], x.w, a, c + 6, &np)) {
+        if (nl_bead(nl, par, par.tab, RecPosG{rec.pos}, RecTypeG{rec.type}, RecChargeG{rec.pos},
+                    NoReaction{}, p.x, p.y, p.z, lig.type[i], x.w, a, c + 6, &np)) {
             pp.x = pp.x + a.x; pp.y = pp.y + a.y; pp.z = pp.z + a.z; pp.w = pp.w + a.w;
         }
         xyz[i] = p.x; xyz[n + i] = p.y; xyz[2 * n + i] = p.z;
@@ -285,7 +327,9 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
     const unsigned nr = rec.n, nL = lig.n;
     for (unsigned j = lane; j < nr; j += kWave) {
         const float4 x = rec.pos[j];
-        const V3 df = deform(rec, j, x.x, x.y, x.z, d + 6);
+        BeadModes md;
+        load_modes(rec, j, md);
+        const V3 df = deform(md, rec.M, x.x, x.y, x.z, d + 6);
         rdefo[j] = df.x; rdefo[nr + j] = df.y; rdefo[2 * nr + j] = df.z;
         V3 p = matvec(Rinv, df.x, df.y, df.z);
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
@@ -296,7 +340,9 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
     }
     for (unsigned i = lane; i < nL; i += kWave) {
         const float4 x = lig.pos[i];
-        const V3 df = deform(lig, i, x.x, x.y, x.z, d + 16);
+        BeadModes md;
+        load_modes(lig, i, md);
+        const V3 df = deform(md, lig.M, x.x, x.y, x.z, d + 16);
         V3 p = matvec(R, df.x, df.y, df.z);
         p.x = p.x + d[0]; p.y = p.y + d[1]; p.z = p.z + d[2];
         xyz[i] = p.x; xyz[nL + i] = p.y; xyz[2 * nL + i] = p.z;
@@ -307,7 +353,7 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
     }
     __syncthreads();
     if (lane != 0) return;
-    auto recPos = [&](unsigned j) { return make_float4(rdefo[j], rdefo[nr + j], rdefo[2 * nr + j], rec.pos[j].w); };
+    auto recPos = [&](unsigned j) { return make_float3(rdefo[j], rdefo[nr + j], rdefo[2 * nr + j]); };
     auto reaction = [&](unsigned j, float fx, float fy, float fz) {
         rpot[j] += fx; rpot[nr + j] += fy; rpot[2 * nr + j] += fz;
     };
@@ -315,8 +361,9 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
         const float4 x = lig.pos[i];
         float4 a;
         unsigned np = 0;
-        if (nl_bead(nl, rec, par, par.tab, recPos, reaction, xyz[i], xyz[nL + i], xyz[2 * nL + i],
-                    lig.type[i], x.w, a, cell + 9 * i + 6, &np)) {
+        if (nl_bead(nl, par, par.tab, recPos, RecTypeG{rec.type}, RecChargeG{rec.pos}, reaction,
+                    xyz[i], xyz[nL + i], xyz[2 * nL + i], lig.type[i], x.w, a, cell + 9 * i + 6,
+                    &np)) {
             pot[i] += a.x; pot[nL + i] += a.y; pot[2 * nL + i] += a.z; pot[3 * nL + i] += a.w;
         }
         npairs[i] = np;
@@ -334,7 +381,58 @@ __global__ void k_debug_rotmats(const float* ang, unsigned n, float* R) {
     rotmat_from_trig((float)c0, (float)c1, (float)s0, (float)s1, (float)c2, (float)s2, R + 9 * i);
 }
 
+// Expands a [grid][z][y][x] float4 grid into voxel octets (see DGrid).
+__global__ __launch_bounds__(256) void k_octets(const float4* __restrict__ g,
+                                                float4* __restrict__ oct, unsigned nx,
+                                                unsigned ny, unsigned nz, unsigned ngrids) {
+    const size_t nvo = (size_t)(nx - 1) * (ny - 1) * (nz - 1);
+    const size_t total = nvo * ngrids;
+    for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total;
+         o += (size_t)gridDim.x * blockDim.x) {
+        size_t r = o;
+        const unsigned ix = r % (nx - 1); r /= (nx - 1);
+        const unsigned iy = r % (ny - 1); r /= (ny - 1);
+        const unsigned iz = r % (nz - 1); r /= (nz - 1);
+        const size_t t = r;
+        const size_t nxy = (size_t)nx * ny;
+        const float4* b = g + t * nxy * nz + nxy * iz + (size_t)nx * iy + ix;
+        float4* d = oct + 8 * o;
+        d[0] = b[0];
+        d[1] = b[1];
+        d[2] = b[nx];
+        d[3] = b[nxy];
+        d[4] = b[nxy + 1];
+        d[5] = b[nxy + nx];
+        d[6] = b[nx + 1];
+        d[7] = b[nxy + nx + 1];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_copy(float4* __restrict__ dst,
+                                              const float4* __restrict__ src, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // ------------------------------------------------------------------ launchers
+
+hipError_t launch_octets(const float4* grid, float4* oct, unsigned nx, unsigned ny, unsigned nz,
+                         unsigned ngrids, hipStream_t s) {
+    const size_t total = (size_t)(nx - 1) * (ny - 1) * (nz - 1) * ngrids;
+    const size_t blocks = std::min<size_t>((total + 255) / 256, 256 * 256);
+    hipLaunchKernelGGL(k_octets, dim3((unsigned)blocks), dim3(256), 0, s, grid, oct, nx, ny, nz,
+                       ngrids);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s) {
+    if (n16 == 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((n16 + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s,
+                       static_cast<float4*>(dst), static_cast<const float4*>(src), n16);
+    return hipGetLastError();
+}
 
 hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, const DGrid& outer,
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
@@ -348,10 +446,12 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
     return hipGetLastError();
 }
 
+// Dynamic LDS of k_modes: pair table, deformed receptor xyz, receptor types (u16).
 size_t modes_lds_bytes(const DProt& rec, const DPar& par, unsigned* tabSlots) {
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
-    return sizeof(att_pair_params) * slots + sizeof(float) * 3 * (size_t)rec.n;
+    return sizeof(att_pair_params) * slots + sizeof(float) * 3 * (size_t)rec.n +
+           sizeof(unsigned short) * (((size_t)rec.n + 1) & ~(size_t)1);
 }
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,

@@ -89,6 +89,7 @@ def lib():
         L.att_event_destroy.argtypes = [P, P]
         L.att_event_record.argtypes = [P, P, P]
         L.att_event_elapsed_ms.argtypes = [P, P, P, C.POINTER(C.c_float)]
+        L.att_debug_copy.argtypes = [P, P, P, C.c_size_t, P]
         _lib = L
     return _lib
 
@@ -232,6 +233,9 @@ class Context:
         out = np.empty(shape, dtype)
         self._chk(self.L.att_memcpy_d2h(self.h, _p(out), buf.ptr, out.nbytes, None))
         return out
+
+    def copy(self, dst, src, nbytes, stream=None):
+        self._chk(self.L.att_debug_copy(self.h, dst.ptr, src.ptr, nbytes, stream))
 
     def stream(self):
         s = P()

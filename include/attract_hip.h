@@ -156,6 +156,12 @@ att_status att_debug_rotmats(att_ctx* ctx, const float* angles /* 3n */, size_t 
 
 att_status att_sync(att_ctx* ctx, void* stream);
 
+/* A plain float4 copy of `bytes` (multiple of 16) from d_src to d_dst on
+ * `stream`: the known-size kernel that calibrates hardware byte counters
+ * (FETCH_SIZE/WRITE_SIZE) against the scoring kernels.  No reference
+ * counterpart. */
+att_status att_debug_copy(att_ctx* ctx, void* d_dst, const void* d_src, size_t bytes, void* stream);
+
 /* --- device buffers, copies and timing -------------------------------------
  * Replace the reference's DeviceAllocator / HostPinnedAllocator
  * (allocator/DeviceAllocator.tpp:19-30, allocator/HostPinnedAllocator.tpp:19-30)
