@@ -310,14 +310,25 @@ __device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
         for (int k = 0; k < 4; ++k) idx[k] = lst[j0 + k < d.x ? j0 + k : j0];
 #pragma unroll
         for (int k = 0; k < 4; ++k) { rp[k] = recPos(idx[k]); rt[k] = recType(idx[k]); }
+        // Distances of the four pairs first, then their plateau ratios, so the
+        // four table loads are in flight together.
+        float ddx[4], ddy[4], ddz[4], dr2s[4], ratios[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ddx[k] = px - rp[k].x;
+            ddy[k] = py - rp[k].y;
+            ddz[k] = pz - rp[k].z;
+            dr2s[k] = ddx[k] * ddx[k] + ddy[k] * ddy[k] + ddz[k] * ddz[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            ratios[k] = (j0 + k < d.x && dr2s[k] <= nl.P2) ? nl.ratio[(int)(dr2s[k] * 10000.f)] : 0.f;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (j0 + k >= d.x) break;
             const unsigned nIdx = idx[k];
-            float dx = px - rp[k].x;
-            float dy = py - rp[k].y;
-            float dz = pz - rp[k].z;
-            const float dr2 = dx * dx + dy * dy + dz * dz;
+            float dx = ddx[k], dy = ddy[k], dz = ddz[k];
+            const float dr2 = dr2s[k];
             if (!(dr2 <= nl.P2)) continue;  // outOfPlateau (model/NLGrid.h:133-135); NaN skipped
             // A listed bead without a valid type would index outside the table
             // (the reference reads out of bounds there); such pairs are skipped.
@@ -331,7 +342,7 @@ __device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
             lj(dr2, dr2_inv, dx, dy, dz, p, par.shape, vx, vy, vz, ve);
             fx += vx; fy += vy; fz += vz; e += ve;
             rfx -= vx; rfy -= vy; rfz -= vz;
-            const float ratio = nl.ratio[(int)(dr2 * 10000.f)];
+            const float ratio = ratios[k];
             const float rdx = ratio * dx, rdy = ratio * dy, rdz = ratio * dz;
             lj(nl.P2, nl.P2_inv, rdx, rdy, rdz, p, par.shape, vx, vy, vz, ve);
             fx -= vx; fy -= vy; fz -= vz; e -= ve;
@@ -353,6 +364,43 @@ __device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
     acc = make_float4(fx, fy, fz, e);
     if (npairs) *npairs = pairs;
     return true;
+}
+
+// Number of listed neighbours nl_bead will traverse for a bead (0 when the
+// bead has no neighbour-list term); same bounds and index as nl_bead.
+__device__ __forceinline__ unsigned nl_count(const DNL& nl, float px, float py, float pz,
+                                             unsigned ligType) {
+    if (ligType == 0) return 0;
+    if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
+          (pz >= nl.loz && pz <= nl.hiz)))
+        return 0;
+    const int ix = (int)roundf((px - nl.lox) * nl.dVox_inv);
+    const int iy = (int)roundf((py - nl.loy) * nl.dVox_inv);
+    const int iz = (int)roundf((pz - nl.loz) * nl.dVox_inv);
+    return nl.desc[nl.nx * (iz * nl.ny + iy) + ix].x;
+}
+
+// Bitonic sort of n (a power of two) keys in LDS into descending order, by the
+// whole workgroup.
+__device__ __forceinline__ void bitonic_sort_desc(unsigned* s, unsigned n) {
+    for (unsigned k = 2; k <= n; k <<= 1)
+        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+            for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
+                const unsigned l = i ^ j;
+                if (l > i) {
+                    const unsigned a = s[i], b = s[l];
+                    const bool swap = ((i & k) == 0) ? (a < b) : (a > b);
+                    if (swap) { s[i] = b; s[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+__host__ __device__ inline unsigned pow2_ceil(unsigned v) {
+    unsigned p = 1;
+    while (p < v) p <<= 1;
+    return p;
 }
 
 // Butterfly sum over the 64 lanes of a wave (same value on every lane).

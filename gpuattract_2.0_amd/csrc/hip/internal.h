@@ -11,7 +11,7 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
                         hipStream_t s);
 
-size_t modes_lds_bytes(const DProt& rec, const DPar& par, unsigned* tabSlots);
+size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots);
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,

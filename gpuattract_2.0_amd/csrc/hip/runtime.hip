@@ -465,7 +465,8 @@ static att_status check_modes(att_ctx* c, ProtObj* rec, ProtObj* lig, GridObj* g
     if ((st = check_pair(c, rec, lig, gr, p)) != ATT_OK) return st;
     if (rec->maxMType >= gl->inner.ngrids || rec->maxMType >= gl->outer.ngrids)
         return fail(c, ATT_ERR_INVALID, "receptor mapped type has no ligand grid");
-    const size_t lds = modes_lds_bytes(rec->d, p->d, nullptr) + sizeof(float) * 4 * 33;
+    if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
+    const size_t lds = modes_lds_bytes(rec->d, lig->d, p->d, nullptr) + sizeof(float) * 4 * 33 + 16;
     if (lds > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "receptor too large for the LDS-staged modes kernel");
     return ATT_OK;

@@ -101,15 +101,18 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
     }
 }
 
-// Mode vectors of bead i (modes stored [c][m][n], coalesced over beads).
+// Mode vectors of bead i (modes stored [c][m][n], coalesced over beads); MM
+// is the compile-time capacity (>= the body's number of modes).
+template <int MM>
 struct BeadModes {
-    float x[ATT_MODES_MAX], y[ATT_MODES_MAX], z[ATT_MODES_MAX];
+    float x[MM > 0 ? MM : 1], y[MM > 0 ? MM : 1], z[MM > 0 ? MM : 1];
 };
 
-__device__ __forceinline__ void load_modes(const DProt& b, unsigned i, BeadModes& md) {
+template <int MM>
+__device__ __forceinline__ void load_modes(const DProt& b, unsigned i, BeadModes<MM>& md) {
     const size_t nM = (size_t)b.M * b.n;
 #pragma unroll
-    for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
+    for (unsigned m = 0; m < (unsigned)MM; ++m) {
         md.x[m] = m < b.M ? b.modes[(size_t)m * b.n + i] : 0.f;
         md.y[m] = m < b.M ? b.modes[nM + (size_t)m * b.n + i] : 0.f;
         md.z[m] = m < b.M ? b.modes[2 * nM + (size_t)m * b.n + i] : 0.f;
@@ -118,11 +121,12 @@ __device__ __forceinline__ void load_modes(const DProt& b, unsigned i, BeadModes
 
 // Mode deformation x + sum_m a_m * mode_m, sequential over modes
 // (model/transform.h:69-73).
-__device__ __forceinline__ V3 deform(const BeadModes& md, unsigned M, float x, float y, float z,
-                                     const float* amp) {
+template <int MM>
+__device__ __forceinline__ V3 deform(const BeadModes<MM>& md, unsigned M, float x, float y,
+                                     float z, const float* amp) {
     V3 p{x, y, z};
 #pragma unroll
-    for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
+    for (unsigned m = 0; m < (unsigned)MM; ++m) {
         if (m < M) {
             const float a = amp[m];
             p.x = p.x + a * md.x[m];
@@ -134,15 +138,17 @@ __device__ __forceinline__ V3 deform(const BeadModes& md, unsigned M, float x, f
 }
 
 // Accumulates -f . mode_m into g[m] (model/reduction_modes.h:49-55, 83-86).
-__device__ __forceinline__ void mode_dots(const BeadModes& md, unsigned M, float fx, float fy,
+template <int MM>
+__device__ __forceinline__ void mode_dots(const BeadModes<MM>& md, unsigned M, float fx, float fy,
                                           float fz, float* g) {
 #pragma unroll
-    for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
+    for (unsigned m = 0; m < (unsigned)MM; ++m)
         if (m < M) g[m] -= fx * md.x[m] + fy * md.y[m] + fz * md.z[m];
 }
 
 constexpr int kModesVals = 13 + 2 * ATT_MODES_MAX;
 
+template <int MM>
 __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter,
                                                DNL nl, DGrid lInner, DGrid lOuter, DPar par,
                                                const float* __restrict__ dofs,
@@ -155,15 +161,18 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     float* s_ry = s_rx + rec.n;
     float* s_rz = s_ry + rec.n;
     unsigned short* s_rt = reinterpret_cast<unsigned short*>(s_rz + rec.n);
+    unsigned* s_queue = reinterpret_cast<unsigned*>(s_rt + ((rec.n + 1) & ~1u));
+    __shared__ unsigned s_qn;
     const unsigned pose = blockIdx.x;
     if (pose >= n) return;  // uniform per workgroup
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const float* d = dofs + 26 * (size_t)pose;
     const float tx = d[0], ty = d[1], tz = d[2];
-    float ampR[ATT_MODES_MAX], ampL[ATT_MODES_MAX];
+    constexpr int MA = MM > 0 ? MM : 1;
+    float ampR[MA], ampL[MA];
 #pragma unroll
-    for (int m = 0; m < ATT_MODES_MAX; ++m) { ampR[m] = d[6 + m]; ampL[m] = d[16 + m]; }
+    for (int m = 0; m < MA; ++m) { ampR[m] = d[6 + m]; ampL[m] = d[16 + m]; }
     float t[6];
     pose_trig(d[3], d[4], d[5], lane, t);
     float R[9], RT[9], Rinv[9];
@@ -177,16 +186,17 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     const V3 pinv = matvec(RT, -tx, -ty, -tz);
 
     float acc[13];
-    float mR[ATT_MODES_MAX], mL[ATT_MODES_MAX];
+    float mR[MA], mL[MA];
 #pragma unroll
     for (int k = 0; k < 13; ++k) acc[k] = 0.f;
 #pragma unroll
-    for (int m = 0; m < ATT_MODES_MAX; ++m) { mR[m] = 0.f; mL[m] = 0.f; }
+    for (int m = 0; m < MA; ++m) { mR[m] = 0.f; mL[m] = 0.f; }
 
+    if (threadIdx.x == 0) s_qn = 0;
     // Phase A: receptor beads.
     for (unsigned j = threadIdx.x; j < rec.n; j += blockDim.x) {
         const float4 x = rec.pos[j];
-        BeadModes md;
+        BeadModes<MM> md;
         load_modes(rec, j, md);
         const V3 df = deform(md, rec.M, x.x, x.y, x.z, ampR);
         s_rx[j] = df.x; s_ry[j] = df.y; s_rz[j] = df.z;
@@ -195,6 +205,7 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
         const float4 pot = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
         const V3 f = matvec(R, pot.x, pot.y, pot.z);  // rotate_forces (transform_modes.h:141-161)
+        load_modes(rec, j, md);  // reloaded (cache hit) rather than held across the lookup
         mode_dots(md, rec.M, f.x, f.y, f.z, mR);
     }
     __syncthreads();
@@ -207,51 +218,81 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     auto recType = [=](unsigned j) { return (unsigned)s_rt[j]; };
     auto reaction = [&](unsigned j, float fx, float fy, float fz) {
         const float4* row = rmodesT + (size_t)j * rowT;
-        float mt[4 * ((3 * ATT_MODES_MAX + 3) / 4)];
+        constexpr unsigned Q = (3 * MA + 3) / 4;
+        float mt[4 * Q];
 #pragma unroll
-        for (unsigned q = 0; q < (3 * ATT_MODES_MAX + 3) / 4; ++q) {
+        for (unsigned q = 0; q < Q; ++q) {
             const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
             mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
         }
 #pragma unroll
-        for (unsigned m = 0; m < ATT_MODES_MAX; ++m)
+        for (unsigned m = 0; m < (unsigned)MM; ++m)
             if (m < Mr) mR[m] -= fx * mt[3 * m] + fy * mt[3 * m + 1] + fz * mt[3 * m + 2];
     };
-    for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
-        const float4 x = lig.pos[i];
-        BeadModes md;
+    // Per-bead sums are linear in the bead force, so the grid part and the
+    // neighbour-list part of each ligand bead are added to them separately.
+    auto add_bead = [&](const float4& x, unsigned i, float fx, float fy, float fz, float e) {
+        acc[0] += e;
+        acc[1] += fx; acc[2] += fy; acc[3] += fz;
+        acc[4] += x.x * fx; acc[5] += x.y * fx; acc[6] += x.z * fx;
+        acc[7] += x.x * fy; acc[8] += x.y * fy; acc[9] += x.z * fy;
+        acc[10] += x.x * fz; acc[11] += x.y * fz; acc[12] += x.z * fz;
+        // reduceModeForce, ligand (model/reduction_modes.h:28-57): R^T f . mode.
+        const V3 fl = matvec(RT, fx, fy, fz);
+        BeadModes<MM> md;
+        load_modes(lig, i, md);
+        mode_dots(md, lig.M, fl.x, fl.y, fl.z, mL);
+    };
+    auto place = [&](unsigned i, float4& x) {
+        x = lig.pos[i];
+        BeadModes<MM> md;
         load_modes(lig, i, md);
         const V3 df = deform(md, lig.M, x.x, x.y, x.z, ampL);
         V3 p = matvec(R, df.x, df.y, df.z);
         p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
-        float4 pot = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
+        return p;
+    };
+    // Phase B1: ligand beads on the receptor grid; beads with a neighbour list
+    // are queued under the key (list length << 16 | bead).
+    for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
+        float4 x;
+        const V3 p = place(i, x);
+        const float4 pot = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
+        add_bead(x, i, pot.x, pot.y, pot.z, pot.w);
+        const unsigned cnt = nl_count(nl, p.x, p.y, p.z, lig.type[i]);
+        if (cnt) s_queue[atomicAdd(&s_qn, 1u)] = (cnt < 0xFFFFu ? cnt : 0xFFFFu) << 16 | i;
+    }
+    __syncthreads();
+    // Phase B2: neighbour lists, longest first, so that the lanes of a wave
+    // walk lists of similar length.  The keys are unique, so the sorted order
+    // does not depend on the order the queue was filled in.
+    const unsigned qn = s_qn;
+    const unsigned qp = pow2_ceil(qn);
+    for (unsigned q = qn + threadIdx.x; q < qp; q += blockDim.x) s_queue[q] = 0;
+    __syncthreads();
+    bitonic_sort_desc(s_queue, qp);
+    for (unsigned q = threadIdx.x; q < qn; q += blockDim.x) {
+        const unsigned i = s_queue[q] & 0xFFFFu;
+        float4 x;
+        const V3 p = place(i, x);
         float4 a;
-        if (nl_bead(nl, par, tab, recPos, recType, RecChargeG{rpos}, reaction, p.x, p.y, p.z,
-                    lig.type[i], x.w, a, nullptr, nullptr)) {
-            pot.x = pot.x + a.x; pot.y = pot.y + a.y; pot.z = pot.z + a.z; pot.w = pot.w + a.w;
-        }
-        acc[0] += pot.w;
-        acc[1] += pot.x; acc[2] += pot.y; acc[3] += pot.z;
-        acc[4] += x.x * pot.x; acc[5] += x.y * pot.x; acc[6] += x.z * pot.x;
-        acc[7] += x.x * pot.y; acc[8] += x.y * pot.y; acc[9] += x.z * pot.y;
-        acc[10] += x.x * pot.z; acc[11] += x.y * pot.z; acc[12] += x.z * pot.z;
-        // reduceModeForce, ligand (model/reduction_modes.h:28-57): R^T f . mode.
-        const V3 fl = matvec(RT, pot.x, pot.y, pot.z);
-        mode_dots(md, lig.M, fl.x, fl.y, fl.z, mL);
+        nl_bead(nl, par, tab, recPos, recType, RecChargeG{rpos}, reaction, p.x, p.y, p.z,
+                lig.type[i], x.w, a, nullptr, nullptr);
+        add_bead(x, i, a.x, a.y, a.z, a.w);
     }
 
     // Workgroup reduction: wave butterflies, then the four waves in order.
 #pragma unroll
     for (int k = 0; k < 13; ++k) acc[k] = wave_sum(acc[k]);
 #pragma unroll
-    for (int m = 0; m < ATT_MODES_MAX; ++m) { mR[m] = wave_sum(mR[m]); mL[m] = wave_sum(mL[m]); }
+    for (int m = 0; m < MM; ++m) { mR[m] = wave_sum(mR[m]); mL[m] = wave_sum(mL[m]); }
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < 13; ++k) s_red[wave][k] = acc[k];
 #pragma unroll
         for (int m = 0; m < ATT_MODES_MAX; ++m) {
-            s_red[wave][13 + m] = mR[m];
-            s_red[wave][13 + ATT_MODES_MAX + m] = mL[m];
+            s_red[wave][13 + m] = m < MM ? mR[m < MM ? m : 0] : 0.f;
+            s_red[wave][13 + ATT_MODES_MAX + m] = m < MM ? mL[m < MM ? m : 0] : 0.f;
         }
     }
     __syncthreads();
@@ -327,7 +368,7 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
     const unsigned nr = rec.n, nL = lig.n;
     for (unsigned j = lane; j < nr; j += kWave) {
         const float4 x = rec.pos[j];
-        BeadModes md;
+        BeadModes<ATT_MODES_MAX> md;
         load_modes(rec, j, md);
         const V3 df = deform(md, rec.M, x.x, x.y, x.z, d + 6);
         rdefo[j] = df.x; rdefo[nr + j] = df.y; rdefo[2 * nr + j] = df.z;
@@ -340,7 +381,7 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
     }
     for (unsigned i = lane; i < nL; i += kWave) {
         const float4 x = lig.pos[i];
-        BeadModes md;
+        BeadModes<ATT_MODES_MAX> md;
         load_modes(lig, i, md);
         const V3 df = deform(md, lig.M, x.x, x.y, x.z, d + 16);
         V3 p = matvec(R, df.x, df.y, df.z);
@@ -446,12 +487,14 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
     return hipGetLastError();
 }
 
-// Dynamic LDS of k_modes: pair table, deformed receptor xyz, receptor types (u16).
-size_t modes_lds_bytes(const DProt& rec, const DPar& par, unsigned* tabSlots) {
+// Dynamic LDS of k_modes: pair table, deformed receptor xyz, receptor types
+// (u16), and the neighbour-list queue (one u32 per ligand bead, power of two).
+size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots) {
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
     return sizeof(att_pair_params) * slots + sizeof(float) * 3 * (size_t)rec.n +
-           sizeof(unsigned short) * (((size_t)rec.n + 1) & ~(size_t)1);
+           sizeof(unsigned short) * (((size_t)rec.n + 1) & ~(size_t)1) +
+           sizeof(unsigned) * (size_t)pow2_ceil(lig.n);
 }
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
@@ -460,9 +503,17 @@ hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         unsigned n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     unsigned slots = 0;
-    const size_t lds = modes_lds_bytes(rec, par, &slots);
-    hipLaunchKernelGGL(k_modes, dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, nl, lInner,
-                       lOuter, par, dofs, res, n, slots);
+    const size_t lds = modes_lds_bytes(rec, lig, par, &slots);
+    const unsigned mm = rec.M > lig.M ? rec.M : lig.M;
+    if (mm == 0)
+        hipLaunchKernelGGL(k_modes<0>, dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, nl,
+                           lInner, lOuter, par, dofs, res, n, slots);
+    else if (mm <= 5)
+        hipLaunchKernelGGL(k_modes<5>, dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, nl,
+                           lInner, lOuter, par, dofs, res, n, slots);
+    else
+        hipLaunchKernelGGL(k_modes<ATT_MODES_MAX>, dim3(n), dim3(256), lds, s, rec, lig, rInner,
+                           rOuter, nl, lInner, lOuter, par, dofs, res, n, slots);
     return hipGetLastError();
 }
 
