@@ -355,6 +355,39 @@ extern "C" int attg_build_grid(const attg_geom* g, const float* xyz, const unsig
     return 0;
 }
 
+extern "C" int attg_build_basis(const attg_geom* g, const float* xyz, const unsigned* type,
+                                unsigned n, unsigned T, int shape, const float* rmin2,
+                                float plateau, float* out, int nthreads) {
+    if (shape != 8 && shape != 12) return -2;
+    const double P2 = (double)plateau * (double)plateau;
+    for (unsigned k = 0; k < T * T; ++k)
+        if ((double)rmin2[k] > P2) return -1;
+    const size_t nx = g->nx, ny = g->ny, nz = g->nz, npts = nx * ny * nz;
+    const float* X = xyz;
+    const float* Y = xyz + n;
+    const float* Z = xyz + 2 * (size_t)n;
+    parallel_for(npts, nthreads, [&](size_t pidx) {
+        const size_t ix = pidx % nx, iy = (pidx / nx) % ny, iz = pidx / (nx * ny);
+        const double px = (double)g->ox + (double)ix * g->spacing;
+        const double py = (double)g->oy + (double)iy * g->spacing;
+        const double pz = (double)g->oz + (double)iz * g->spacing;
+        double acc[2 * 100];
+        std::fill(acc, acc + 2 * T, 0.0);
+        for (unsigned j = 0; j < n; ++j) {
+            const unsigned a = type[j];
+            if (a == 0 || a > T) continue;
+            const double dx = px - X[j], dy = py - Y[j], dz = pz - Z[j];
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            const double i2 = 1.0 / (r2 > P2 ? r2 : P2);
+            const double i6 = i2 * i2 * i2;
+            acc[2 * (a - 1)] += shape == 8 ? i6 * i2 : i6 * i6;
+            acc[2 * (a - 1) + 1] += i6;
+        }
+        for (unsigned b = 0; b < 2 * T; ++b) out[b * npts + pidx] = (float)acc[b];
+    });
+    return 0;
+}
+
 extern "C" int attg_build_nl(const attg_geom* g, const float* xyz, const unsigned* type,
                              unsigned n, float neighbourdis, uint32_t* desc, uint32_t* list,
                              uint64_t* total, int nthreads) {

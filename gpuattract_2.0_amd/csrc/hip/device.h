@@ -49,6 +49,14 @@ struct DProt {
     unsigned n, M, rowT;  // rowT = float4 per modesT row
 };
 
+// Basis grids of the parameter derivative (attached to a DGrid of the same
+// geometry): per voxel, per basis channel, the 8 corner values in trilinear
+// order (two float4).  nb = 2T channels [S_1, S6_1, S_2, S6_2, ...].
+struct DBasis {
+    const float4* oct;
+    unsigned nb;
+};
+
 struct DPar {
     const att_pair_params* tab;  // T*T, index T*(recType-1)+(ligType-1)
     unsigned T;
@@ -428,6 +436,82 @@ __device__ __forceinline__ float correct_mode(float d, float k) {
     const double dd = (double)d, d2 = dd * dd;
     const float cf = (float)((double)(4.f * k) * (d2 * d2));
     return d + cf;
+}
+
+// Pair table into LDS when it fits (kMaxTypesLDS), else read from global.
+__device__ __forceinline__ const att_pair_params* stage_table(const DPar& par,
+                                                              att_pair_params* s_tab) {
+    if (par.T <= kMaxTypesLDS) {
+        for (unsigned k = threadIdx.x; k < par.T * par.T; k += blockDim.x) s_tab[k] = par.tab[k];
+        return s_tab;
+    }
+    return par.tab;
+}
+
+struct NoReaction {
+    __device__ void operator()(unsigned, float, float, float) const {}
+};
+
+// Receptor accessors reading the protein arrays in global memory.
+struct RecPosG {
+    const float4* pos;
+    __device__ float3 operator()(unsigned j) const {
+        const float4 v = pos[j];
+        return make_float3(v.x, v.y, v.z);
+    }
+};
+struct RecTypeG {
+    const unsigned* type;
+    __device__ unsigned operator()(unsigned j) const { return type[j]; }
+};
+struct RecChargeG {
+    const float4* pos;
+    __device__ float operator()(unsigned j) const { return pos[j].w; }
+};
+
+// Mode vectors of bead i (modes stored [c][m][n], coalesced over beads); MM
+// is the compile-time capacity (>= the body's number of modes).
+template <int MM>
+struct BeadModes {
+    float x[MM > 0 ? MM : 1], y[MM > 0 ? MM : 1], z[MM > 0 ? MM : 1];
+};
+
+template <int MM>
+__device__ __forceinline__ void load_modes(const DProt& b, unsigned i, BeadModes<MM>& md) {
+    const size_t nM = (size_t)b.M * b.n;
+#pragma unroll
+    for (unsigned m = 0; m < (unsigned)MM; ++m) {
+        md.x[m] = m < b.M ? b.modes[(size_t)m * b.n + i] : 0.f;
+        md.y[m] = m < b.M ? b.modes[nM + (size_t)m * b.n + i] : 0.f;
+        md.z[m] = m < b.M ? b.modes[2 * nM + (size_t)m * b.n + i] : 0.f;
+    }
+}
+
+// Mode deformation x + sum_m a_m * mode_m, sequential over modes
+// (model/transform.h:69-73).
+template <int MM>
+__device__ __forceinline__ V3 deform(const BeadModes<MM>& md, unsigned M, float x, float y,
+                                     float z, const float* amp) {
+    V3 p{x, y, z};
+#pragma unroll
+    for (unsigned m = 0; m < (unsigned)MM; ++m) {
+        if (m < M) {
+            const float a = amp[m];
+            p.x = p.x + a * md.x[m];
+            p.y = p.y + a * md.y[m];
+            p.z = p.z + a * md.z[m];
+        }
+    }
+    return p;
+}
+
+// Accumulates -f . mode_m into g[m] (model/reduction_modes.h:49-55, 83-86).
+template <int MM>
+__device__ __forceinline__ void mode_dots(const BeadModes<MM>& md, unsigned M, float fx, float fy,
+                                          float fz, float* g) {
+#pragma unroll
+    for (unsigned m = 0; m < (unsigned)MM; ++m)
+        if (m < M) g[m] -= fx * md.x[m] + fy * md.y[m] + fz * md.z[m];
 }
 
 }  // namespace att

@@ -36,4 +36,19 @@ hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s);
 hipError_t launch_octets(const float4* grid, float4* oct, unsigned nx, unsigned ny, unsigned nz,
                          unsigned ngrids, hipStream_t s);
 
+size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool pgrad,
+                         unsigned* tabSlots);
+
+hipError_t launch_adjoint(const DProt& rec, const DProt& lig, const DGrid& inner,
+                          const DGrid& outer, const DNL& nl, const DPar& par, const DBasis& bin,
+                          const DBasis& bout, const float* dofs, bool modes, const float* w,
+                          float* out, float* pgPartial, unsigned blocks, unsigned n,
+                          hipStream_t s);
+
+hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, float* pg,
+                               hipStream_t s);
+
+hipError_t launch_basis_octets(const float* b, float4* oct, unsigned nx, unsigned ny, unsigned nz,
+                               unsigned nb, hipStream_t s);
+
 }  // namespace att

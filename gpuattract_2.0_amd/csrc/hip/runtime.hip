@@ -46,6 +46,7 @@ struct ProtObj : Obj {
 struct GridObj : Obj {
     GridObj() : Obj(Kind::Grid) {}
     DGrid inner{}, outer{};
+    DBasis bin{}, bout{};
     DNL nl{};
     bool hasNL = false;
     uint32_t nlMaxIndex = 0;
@@ -71,6 +72,13 @@ struct att_ctx {
     void* d_out = nullptr;
     size_t d_out_cap = 0;
     size_t maxLdsPerBlock = 65536;
+    unsigned numCU = 256;
+    void* d_pg = nullptr;  // per-workgroup parameter-gradient rows of the adjoint
+    size_t d_pg_cap = 0;
+    void* d_w = nullptr;   // staging of host-buffer adjoint calls
+    size_t d_w_cap = 0;
+    void* d_g = nullptr;
+    size_t d_g_cap = 0;
 };
 
 namespace {
@@ -221,6 +229,9 @@ att_status att_ctx_create(int hip_device, att_ctx** out) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, hip_device) == hipSuccess && lds > 0)
         c->maxLdsPerBlock = (size_t)lds;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hip_device) == hipSuccess && cus > 0)
+        c->numCU = (unsigned)cus;
     *out = c.release();
     return ATT_OK;
 }
@@ -232,6 +243,9 @@ void att_ctx_destroy(att_ctx* c) {
     c->objs.clear();
     if (c->d_in) (void)hipFree(c->d_in);
     if (c->d_out) (void)hipFree(c->d_out);
+    if (c->d_pg) (void)hipFree(c->d_pg);
+    if (c->d_w) (void)hipFree(c->d_w);
+    if (c->d_g) (void)hipFree(c->d_g);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -402,6 +416,16 @@ static att_status resolve(att_ctx* c, att_handle hr, att_handle hl, att_handle h
     return ATT_OK;
 }
 
+static att_status grow(att_ctx* c, void** p, size_t* cap, size_t bytes) {
+    if (bytes <= *cap) return ATT_OK;
+    if (*p) ATT_HIP(c, hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+    ATT_HIP(c, hipMalloc(p, bytes));
+    *cap = bytes;
+    return ATT_OK;
+}
+
 static att_status ensure_staging(att_ctx* c, size_t inBytes, size_t outBytes) {
     if (inBytes > c->d_in_cap) {
         if (c->d_in) ATT_HIP(c, hipFree(c->d_in));
@@ -514,6 +538,106 @@ att_status att_score_modes(att_ctx* c, att_handle hr, att_handle hl, att_handle 
     return ATT_OK;
 }
 
+
+att_status att_upload_basis(att_ctx* c, att_handle hg, unsigned T, const float* inner,
+                            const float* outer) {
+    if (!c || !inner || !outer) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    GridObj* g = lookup<GridObj>(c, hg, Kind::Grid);
+    if (!g) return fail(c, ATT_ERR_HANDLE, "unknown grid handle");
+    if (T == 0 || T > 99) return fail(c, ATT_ERR_INVALID, "number of types must be 1..99");
+    const DGrid* gs[2] = {&g->inner, &g->outer};
+    const float* src[2] = {inner, outer};
+    DBasis* dst[2] = {&g->bin, &g->bout};
+    for (int k = 0; k < 2; ++k) {
+        const DGrid& G = *gs[k];
+        const size_t nv = (size_t)G.nx * G.ny * G.nz;
+        const size_t nvo = (size_t)(G.nx - 1) * (G.ny - 1) * (G.nz - 1);
+        void* tmp = nullptr;
+        void* oct = nullptr;
+        ATT_HIP(c, hipMalloc(&oct, sizeof(float4) * 2 * nvo * 2 * T));
+        g->allocs.push_back(oct);
+        hipError_t e = hipMalloc(&tmp, sizeof(float) * nv * 2 * T);
+        if (e == hipSuccess) e = hipMemcpy(tmp, src[k], sizeof(float) * nv * 2 * T, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = launch_basis_octets(static_cast<const float*>(tmp), static_cast<float4*>(oct), G.nx,
+                                    G.ny, G.nz, 2 * T, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (tmp) (void)hipFree(tmp);
+        if (e != hipSuccess) return fail(c, ATT_ERR_HIP, std::string("basis upload: ") + hipGetErrorString(e));
+        dst[k]->oct = static_cast<const float4*>(oct);
+        dst[k]->nb = 2 * T;
+    }
+    return ATT_OK;
+}
+
+att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle hg,
+                                 att_handle hp, const float* d_dofs, int modes, size_t n,
+                                 const float* d_w, float* d_out, float* d_pgrad, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!d_dofs || !d_out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    if (n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one call");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* g;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
+    if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
+    if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
+    const bool pgrad = d_pgrad != nullptr;
+    if (pgrad && (!g->bin.oct || g->bin.nb != 2 * p->d.T))
+        return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
+    const size_t lds = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, nullptr) +
+                       sizeof(float) * 4 * 33 + 16;
+    if (lds > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "receptor too large for the LDS-staged adjoint kernel");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const unsigned cols = 3 * p->d.T * p->d.T;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, (size_t)c->numCU * 3));
+    if (pgrad && (st = grow(c, &c->d_pg, &c->d_pg_cap, sizeof(float) * (size_t)blocks * cols)) != ATT_OK)
+        return st;
+    if (n == 0) {
+        if (pgrad) ATT_HIP(c, hipMemsetAsync(d_pgrad, 0, sizeof(float) * cols, s));
+        return ATT_OK;
+    }
+    ATT_HIP(c, launch_adjoint(rec->d, lig->d, g->inner, g->outer, g->nl, p->d, g->bin, g->bout,
+                              d_dofs, modes != 0, d_w, d_out,
+                              pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks, (unsigned)n, s));
+    if (pgrad)
+        ATT_HIP(c, launch_pgrad_reduce(static_cast<const float*>(c->d_pg), blocks, cols, d_pgrad, s));
+    return ATT_OK;
+}
+
+att_status att_score_adjoint(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                             const float* dofs, int modes, size_t n, const float* w, float* out,
+                             float* pgrad, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!dofs || !out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ParObj* p = lookup<ParObj>(c, hp, Kind::Params);
+    if (!p) return fail(c, ATT_ERR_HANDLE, "unknown parameter handle");
+    const size_t ds = modes ? 26 : 6, os = modes ? 27 : 7;
+    const size_t cols = 3 * (size_t)p->d.T * p->d.T;
+    att_status st;
+    if ((st = ensure_staging(c, std::max<size_t>(n, 1) * ds * sizeof(float),
+                             std::max<size_t>(n, 1) * os * sizeof(float))) != ATT_OK)
+        return st;
+    if (w && (st = grow(c, &c->d_w, &c->d_w_cap, std::max<size_t>(n, 1) * sizeof(float))) != ATT_OK) return st;
+    if (pgrad && (st = grow(c, &c->d_g, &c->d_g_cap, cols * sizeof(float))) != ATT_OK) return st;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (n) ATT_HIP(c, hipMemcpyAsync(c->d_in, dofs, n * ds * sizeof(float), hipMemcpyHostToDevice, s));
+    if (w && n) ATT_HIP(c, hipMemcpyAsync(c->d_w, w, n * sizeof(float), hipMemcpyHostToDevice, s));
+    if ((st = att_score_adjoint_dev(c, hr, hl, hg, hp, static_cast<const float*>(c->d_in), modes, n,
+                                    w ? static_cast<const float*>(c->d_w) : nullptr,
+                                    static_cast<float*>(c->d_out),
+                                    pgrad ? static_cast<float*>(c->d_g) : nullptr, s)) != ATT_OK)
+        return st;
+    if (n) ATT_HIP(c, hipMemcpyAsync(out, c->d_out, n * os * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (pgrad) ATT_HIP(c, hipMemcpyAsync(pgrad, c->d_g, cols * sizeof(float), hipMemcpyDeviceToHost, s));
+    ATT_HIP(c, hipStreamSynchronize(s));
+    return ATT_OK;
+}
 
 att_status att_debug_beads_rigid(att_ctx* c, att_handle hr, att_handle hl, att_handle hg,
                                  att_handle hp, const att_dof6* dof, float* xyz, float* pot,

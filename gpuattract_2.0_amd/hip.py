@@ -90,6 +90,9 @@ def lib():
         L.att_event_record.argtypes = [P, P, P]
         L.att_event_elapsed_ms.argtypes = [P, P, P, C.POINTER(C.c_float)]
         L.att_debug_copy.argtypes = [P, P, P, C.c_size_t, P]
+        L.att_upload_basis.argtypes = [P, H, C.c_uint, P, P]
+        L.att_score_adjoint.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, P, P, P, P]
+        L.att_score_adjoint_dev.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, P, P, P, P]
         _lib = L
     return _lib
 
@@ -208,6 +211,29 @@ class Context:
         self._chk(self.L.att_score_modes(self.h, hs["rec"], hs["lig"], hs["grid"], hs["lig_grid"],
                                          hs["params"], _p(d), d.shape[0], _p(out), None))
         return out
+
+    def upload_basis(self, grid_handle, wl):
+        self._chk(self.L.att_upload_basis(self.h, grid_handle, wl.params.T,
+                                          _p(wl.rec_inner_basis), _p(wl.rec_outer_basis)))
+
+    def adjoint(self, hs, dofs, w=None, pgrad=False, T=None):
+        """Exact gradient of the returned energy: (n, 7|27) {E, w*dE/dpos,
+        w*dE/dang[, w*dE/dmodesRec[10], w*dE/dmodesLig[10]]} and optionally the
+        (T*T, 3) parameter gradient."""
+        d = np.ascontiguousarray(dofs, np.float32)
+        modes = d.shape[1] == 26
+        out = np.zeros((d.shape[0], 27 if modes else 7), np.float32)
+        ww = None if w is None else np.ascontiguousarray(w, np.float32)
+        pg = np.zeros((T * T, 3), np.float32) if pgrad else None
+        self._chk(self.L.att_score_adjoint(self.h, hs["rec"], hs["lig"], hs["grid"], hs["params"],
+                                           _p(d), 1 if modes else 0, d.shape[0], _p(ww), _p(out),
+                                           _p(pg), None))
+        return (out, pg) if pgrad else out
+
+    def adjoint_dev(self, hs, d_dofs, modes, n, d_w, d_out, d_pgrad=None, stream=None):
+        self._chk(self.L.att_score_adjoint_dev(self.h, hs["rec"], hs["lig"], hs["grid"],
+                                               hs["params"], d_dofs, 1 if modes else 0, n, d_w,
+                                               d_out, d_pgrad, stream))
 
     def score_modes_dev(self, hs, d_dofs, n, d_out, stream=None):
         self._chk(self.L.att_score_modes_dev(self.h, hs["rec"], hs["lig"], hs["grid"],

@@ -70,8 +70,11 @@ def _nl_bytes(nl, p, typ):
     return 8.0 * inb.sum(axis=1) + 4.0 * (K * inb).sum(axis=1)
 
 
-def bytes_per_pose(wl, sample=256, seed=0):
-    """Mean algorithmic bytes per pose over a sample of the workload's poses."""
+def bytes_per_pose(wl, sample=256, seed=0, adjoint=False):
+    """Mean algorithmic bytes per pose over a sample of the workload's poses.
+    adjoint=True counts the adjoint's reads: the receptor beads are deformed
+    for the neighbour list but not interpolated on the ligand grid (that term
+    is not part of the energy)."""
     rng = np.random.default_rng(seed)
     n = wl.dofs.shape[0]
     pick = rng.choice(n, size=min(sample, n), replace=False)
@@ -96,7 +99,8 @@ def bytes_per_pose(wl, sample=256, seed=0):
         pinv = -np.einsum("pji,pj->pi", R, t)  # R^T (-t)
         pr = np.einsum("pij,pnj->pni", Rinv, xr) + pinv[:, None, :]
         B = B + 20.0 * rec.n + 12.0 * rec.n * rec.M
-        B = B + _grid_bytes(wl.lig_inner, wl.lig_outer, pr, rec.mtype, rec.charge)
+        if not adjoint:
+            B = B + _grid_bytes(wl.lig_inner, wl.lig_outer, pr, rec.mtype, rec.charge)
         per_pose_io += 2 * 4.0 * (rec.M + lig.M)
     B = B + per_pose_io
     return float(np.mean(B)), float(np.std(B))

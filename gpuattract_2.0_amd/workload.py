@@ -65,6 +65,9 @@ def gen_lib():
         L.attg_build_grid.argtypes = [C.POINTER(_Geom), P, P, P, C.c_uint, C.c_uint, C.c_int,
                                       P, P, P, P, P, C.c_int, C.c_float, C.c_int, C.c_float,
                                       P, C.c_int]
+        L.attg_build_basis.restype = C.c_int
+        L.attg_build_basis.argtypes = [C.POINTER(_Geom), P, P, C.c_uint, C.c_uint, C.c_int, P,
+                                       C.c_float, P, C.c_int]
         L.attg_build_nl.restype = C.c_int
         L.attg_build_nl.argtypes = [C.POINTER(_Geom), P, P, C.c_uint, C.c_float, P, P,
                                     C.POINTER(C.c_uint64), C.c_int]
@@ -170,6 +173,8 @@ class Workload:
     lig_inner: IGrid = None
     lig_outer: IGrid = None
     lig_nl: NLGrid = None
+    rec_inner_basis: np.ndarray = None  # (2T, nz, ny, nx) flattened, for parameter gradients
+    rec_outer_basis: np.ndarray = None
     meta: dict = dataclasses.field(default_factory=dict)
 
     @property
@@ -246,6 +251,18 @@ def build_grid(geom, src, params, src_is_ligand, plateau=10.0):
     return IGrid(geom, T + 1, data)
 
 
+def build_basis(geom, src, params, plateau=10.0):
+    """Basis grids (2*T, nz, ny, nx) of the receptor type sums S_a, S6_a."""
+    T = params.T
+    out = np.zeros(2 * T * geom.nvox, np.float32)
+    g = geom.c()
+    rc = gen_lib().attg_build_basis(C.byref(g), _p(src.xyz), _p(src.type), src.n, T,
+                                    params.shape, _p(params.rmin2), plateau, _p(out), _threads())
+    if rc != 0:
+        raise ValueError("basis grids need rmin^2 <= plateau^2 for every pair")
+    return out
+
+
 def build_nl(geom, src, neighbourdis=10.8, plateau=10.0):
     L = gen_lib()
     desc = np.zeros(geom.nvox * 2, np.uint32)
@@ -281,7 +298,8 @@ def scaled_radius(n, ref):
 def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_types=12,
                   spacing=0.875, margin=10.0, outer_spacing=2.6, plateau=10.0,
                   neighbourdis=10.8, charged_frac=0.2, type0_frac=0.0, shell=None,
-                  dielec=1, shape=8, with_lig_grid=None, repulsive_frac=0.2):
+                  dielec=1, shape=8, with_lig_grid=None, repulsive_frac=0.2,
+                  with_basis=False):
     """A receptor-ligand pair with grids and poses, as described by the
     benchmark configurations (receptor density of 1000 beads per 25 A sphere,
     ligand 300 per 15 A sphere, 3 A minimum spacing, 12 types, 20% charged)."""
@@ -315,6 +333,9 @@ def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_ty
         dof[:, 6:6 + M_rec] = make_amplitudes(seed, n_poses, M_rec, entity=5)
         dof[:, 16:16 + M_lig] = make_amplitudes(seed, n_poses, M_lig, entity=6)
         wl.dofs = dof
+    if with_basis:
+        wl.rec_inner_basis = build_basis(gi, rec, params, plateau)
+        wl.rec_outer_basis = build_basis(go, rec, params, plateau)
     wl.meta = dict(seed=seed, n_rec=n_rec, n_lig=n_lig, n_poses=n_poses, M_rec=M_rec,
                    M_lig=M_lig, R_rec=R_rec, R_lig=R_lig, shell=shell, spacing=spacing,
                    outer_spacing=outer_spacing, plateau=plateau, neighbourdis=neighbourdis)

@@ -97,6 +97,17 @@ int attg_build_grid(const attg_geom* g, const float* xyz, const unsigned* type,
                     const float* emin, int src_is_ligand, float plateau, int dielec,
                     float ffelec, float* out, int nthreads);
 
+/* Basis grids of the pair-parameter derivative: for every source type a
+ * (1..num_types) the energy sums S_a = sum_j 1/r'^s (s = 8 or 12) and
+ * S6_a = sum_j 1/r'^6 over source beads of type a, r' = max(r, plateau), so
+ * that a probe grid's energy is sum_a ipon(rc S_a - ac S6_a) whenever every
+ * rmin^2 <= plateau^2.  out holds 2*num_types*nz*ny*nx floats, ordered
+ * [S_1, S6_1, S_2, S6_2, ...][z][y][x].  Returns -1 when some pair has
+ * rmin^2 > plateau^2 (the grid is then not linear in rc, ac). */
+int attg_build_basis(const attg_geom* g, const float* xyz, const unsigned* type, unsigned n,
+                     unsigned num_types, int shape, const float* rmin2, float plateau,
+                     float* out, int nthreads);
+
 /* Neighbour lists over the voxels of `g`: source beads with type != 0 within
  * `neighbourdis` of the voxel point, ascending bead index.  Call with list ==
  * NULL to get the total count in *total (desc is filled either way). */

@@ -133,6 +133,27 @@ att_status att_score_modes_dev(att_ctx* ctx, att_handle rec, att_handle lig, att
                                att_handle ligGrid, att_handle params, const att_dof6m* d_dofs,
                                size_t n, att_result6m* d_out, void* stream);
 
+/* --- hand-written adjoint -----------------------------------------------------
+ * No reference counterpart (the reference has no gradient beyond its force
+ * reduction; service/CPUEnergyService6D.tpp:100-195 defines the energy).
+ * Exact gradient of the returned energy E = grid term of the ligand beads +
+ * neighbour-list pair terms:
+ *   out per pose (stride 7 rigid, 27 modes): {E, w*dE/dpos[3], w*dE/dang[3],
+ *   w*dE/dmodesRec[10], w*dE/dmodesLig[10]} -- gradients, not forces, with the
+ *   rotation derivative over deformed coordinates.  Per-bead forces are
+ *   recomputed, never stored.
+ *   pgrad (NULL to skip): numTypes^2 x 3 {dE/drc, dE/dac, dE/demin} per table
+ *   entry, summed over poses with weights w (NULL = 1); needs basis grids.
+ * `modes` selects the DOF layout: 0 = att_dof6, 1 = att_dof6m. */
+att_status att_upload_basis(att_ctx* ctx, att_handle grid, unsigned numTypes,
+                            const float* inner /* 2T*nz*ny*nx */, const float* outer);
+att_status att_score_adjoint(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                             att_handle params, const float* dofs, int modes, size_t n,
+                             const float* w, float* out, float* pgrad, void* stream);
+att_status att_score_adjoint_dev(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                                 att_handle params, const float* d_dofs, int modes, size_t n,
+                                 const float* d_w, float* d_out, float* d_pgrad, void* stream);
+
 /* --- diagnostics -------------------------------------------------------------
  * Per-bead intermediates of one pose, computed by the same device functions
  * as the scoring kernels: transformed coordinates (SoA 3n), per-bead

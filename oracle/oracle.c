@@ -24,14 +24,34 @@
 #define REAL float
 #define SFX _f32
 #include "oracle_impl.inc"
+#include "oracle_adj.inc"
 #undef REAL
 #undef SFX
 
 #define REAL double
 #define SFX _f64
 #include "oracle_impl.inc"
+#include "oracle_adj.inc"
 #undef REAL
 #undef SFX
+
+
+/* Exact adjoint (oracle_adj.inc), FP32 and FP64 arithmetic. */
+int orc_adjoint_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                    const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                    const float* inner_basis, const float* outer_basis, const float* dof,
+                    size_t n, const double* w, double* out, double* scale, double* pgrad) {
+    adjoint_f32(rec, lig, inner, outer, nl, par, inner_basis, outer_basis, dof, n, w, out, scale, pgrad);
+    return 0;
+}
+
+int orc_adjoint_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                    const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                    const float* inner_basis, const float* outer_basis, const float* dof,
+                    size_t n, const double* w, double* out, double* scale, double* pgrad) {
+    adjoint_f64(rec, lig, inner, outer, nl, par, inner_basis, outer_basis, dof, n, w, out, scale, pgrad);
+    return 0;
+}
 
 unsigned orc_ratio_size(float plateau) {
     const float P2 = plateau * plateau;

@@ -108,6 +108,27 @@ int orc_beads_modes(const orc_body* rec, const orc_body* lig, const orc_igrid* r
                     const orc_igrid* ligInner, const orc_igrid* ligOuter, const orc_params* par,
                     const float* dof, orc_beads* out);
 
+/* Exact gradient of the energy the scoring functions return (FP64; the _f32
+ * variant computes in FP32 from bit-identical bead positions):
+ *   E = sum over ligand beads of the interpolated grid energy (type grid plus
+ *       charge * electrostatic grid) + sum over listed pairs inside the plateau
+ *       of [LJ(r) - LJ(P)] + [Coulomb(r) - Coulomb(P)].
+ * dof = n*26 (the DOF_6D_Modes layout; amplitudes beyond a body's modes are
+ * ignored), w = per-pose upstream weights (NULL = 1).  out = n*27
+ * {E, dE/dpos, dE/dang, dE/dmodesRec[10], dE/dmodesLig[10]} (gradients times
+ * w); scale (may be NULL, n*27) the sums of |terms| behind each component.
+ * pgrad (may be NULL) = T*T*3 {dE/drc, dE/dac, dE/demin} summed over
+ * poses with weights; the grid part needs the basis grids (attg_build_basis,
+ * 2T floats per grid point [S_1, S6_1, ...][z][y][x]); pass NULL to omit it. */
+int orc_adjoint_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                    const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                    const float* inner_basis, const float* outer_basis, const float* dof,
+                    size_t n, const double* w, double* out, double* scale, double* pgrad);
+int orc_adjoint_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                    const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                    const float* inner_basis, const float* outer_basis, const float* dof,
+                    size_t n, const double* w, double* out, double* scale, double* pgrad);
+
 #ifdef __cplusplus
 }
 #endif

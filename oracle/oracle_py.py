@@ -109,6 +109,11 @@ def oracle_lib():
                                       C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(OrcParamsC), P,
                                       C.POINTER(BeadsC)]
+        for fn in (L.orc_adjoint_f64, L.orc_adjoint_f32):
+            fn.restype = C.c_int
+            fn.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                           C.POINTER(IGridC), C.POINTER(NLGridC),
+                           C.POINTER(OrcParamsC), P, P, P, C.c_size_t, P, P, P, P]
         L.orc_ratio_size.restype = C.c_uint
         L.orc_ratio_size.argtypes = [C.c_float]
         L.orc_ratio_lut.restype = None
@@ -192,6 +197,43 @@ def score_modes(wl, dofs=None, nthreads=1, f64=False):
     res = np.zeros((n, 27), np.float32)
     L.orc_score_modes(*args, _p(res), nthreads)
     return res
+
+
+def as_modes_dofs(wl, dofs=None):
+    """DOFs in the 26-float DOF_6D_Modes layout (rigid DOFs padded with zeros)."""
+    d = np.asarray(wl.dofs if dofs is None else dofs, np.float32)
+    if d.shape[1] == 26:
+        return np.ascontiguousarray(d)
+    out = np.zeros((d.shape[0], 26), np.float32)
+    out[:, :6] = d
+    return out
+
+
+def adjoint(wl, dofs=None, w=None, pgrad=False, scale=False, f32=False):
+    """Exact gradient of the returned energy: (n, 27) {E, dE/dpos, dE/dang,
+    dE/dmodesRec[10], dE/dmodesLig[10]} times w, and optionally the (T*T, 3)
+    parameter gradient {dE/drc, dE/dac, dE/demin} summed over poses.  FP64
+    arithmetic, or FP32 with f32=True (same bead positions as the FP32
+    forward); results are returned as float64 either way."""
+    L = oracle_lib()
+    d = as_modes_dofs(wl, dofs)
+    n = d.shape[0]
+    out = np.zeros((n, 27), np.float64)
+    ww = None if w is None else np.ascontiguousarray(w, np.float64)
+    pg = np.zeros((wl.params.T ** 2, 3), np.float64) if pgrad else None
+    sc = np.zeros((n, 27), np.float64) if scale else None
+    fn = L.orc_adjoint_f32 if f32 else L.orc_adjoint_f64
+    fn(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                      C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                      C.byref(nl_c(wl.rec_nl)), C.byref(params_c(wl.params)),
+                      _p(wl.rec_inner_basis), _p(wl.rec_outer_basis), _p(d), n, _p(ww), _p(out),
+                      _p(sc), _p(pg))
+    ret = (out,)
+    if scale:
+        ret += (sc,)
+    if pgrad:
+        ret += (pg,)
+    return ret if len(ret) > 1 else out
 
 
 def beads(wl, dof):

@@ -1,0 +1,138 @@
+"""HIP adjoint vs the oracle's FP64 adjoint (needs a GPU).
+
+Tolerance: per-pose gradients within 1e-5 of the condition scale (sum of
+|terms|, oracle FP64) against the oracle's FP32 adjoint, which forms the bead
+positions bit-identically; against FP64 the GPU error is bounded by 1.1x the
+FP32 restatement's own error + 1e-6 (the FP32 rounding of positions in steep
+grid regions, which no summation order removes).  Parameter gradients (float
+atomics over many terms) within 1e-4 of the largest entry.  The adjoint's energy equals the forward's
+energy up to summation order.  The torch harness is checked against central
+finite differences of the GPU energy and against the adjoint directly.
+"""
+import numpy as np
+import pytest
+
+from parity import TOL, cond_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from gpuattract_amd import hip
+    c = hip.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def wl_adj_rigid(W):
+    return W.make_workload("adj rigid", 51, 400, 120, 300, with_basis=True)
+
+
+@pytest.fixture(scope="module")
+def wl_adj_modes(W):
+    return W.make_workload("adj modes", 52, 360, 110, 200, 3, 4, with_basis=True)
+
+
+def _upload(ctx, wl):
+    hs = ctx.upload_workload(wl)
+    ctx.upload_basis(hs["grid"], wl)
+    return hs
+
+
+@pytest.mark.parametrize("which", ["rigid", "modes"])
+def test_adjoint_against_oracle(ctx, O, which, wl_adj_rigid, wl_adj_modes):
+    wl = wl_adj_rigid if which == "rigid" else wl_adj_modes
+    hs = _upload(ctx, wl)
+    rng = np.random.default_rng(3)
+    w = rng.uniform(-1.5, 2.0, wl.dofs.shape[0]).astype(np.float32)
+    g, pg = ctx.adjoint(hs, wl.dofs, w=w, pgrad=True, T=wl.params.T)
+    ref, sc, pref = O.adjoint(wl, w=w.astype(np.float64), pgrad=True, scale=True)
+    r32 = O.adjoint(wl, w=w.astype(np.float64), f32=True)
+    cols = list(range(7)) if which == "rigid" else [c for c in range(27) if sc[:, c].max() > 0]
+    e32 = cond_err(g[:, cols], r32[:, cols], sc[:, cols])
+    assert e32.max() <= TOL, e32.max(axis=0)
+    e_gpu = cond_err(g[:, cols], ref[:, cols], sc[:, cols])
+    e_ref = cond_err(r32[:, cols], ref[:, cols], sc[:, cols])
+    print(f"{which}: GPU vs FP64 max {e_gpu.max():.2e}, FP32 oracle vs FP64 max {e_ref.max():.2e}")
+    assert e_gpu.max() <= 1.1 * e_ref.max() + 1e-6
+    scale = np.abs(pref).max()
+    assert np.abs(pg - pref).max() <= 1e-4 * scale
+    # the adjoint's energy is the forward's energy
+    fwd = ctx.score_modes(hs, wl.dofs) if wl.modes else ctx.score_rigid(hs, wl.dofs)
+    o64, s64 = (O.score_modes if wl.modes else O.score_rigid)(wl, f64=True)
+    assert cond_err(g[:, 0], fwd[:, 0], s64[:, 0]).max() <= TOL
+
+
+def test_adjoint_without_parameters_and_device_path(ctx, wl_adj_modes):
+    wl = wl_adj_modes
+    hs = _upload(ctx, wl)
+    host = ctx.adjoint(hs, wl.dofs)
+    n = wl.dofs.shape[0]
+    d_in = ctx.to_device(wl.dofs)
+    d_out = ctx.dev_alloc(n * 27 * 4)
+    ctx.adjoint_dev(hs, d_in.ptr, True, n, None, d_out.ptr)
+    assert np.array_equal(ctx.to_host(d_out, (n, 27)), host)
+    # parameter gradients need the basis grids: a grid without them is refused
+    from gpuattract_amd import hip
+    hs2 = ctx.upload_workload(wl)
+    with pytest.raises(hip.AttError):
+        ctx.adjoint(hs2, wl.dofs, pgrad=True, T=wl.params.T)
+
+
+def test_adjoint_deterministic(ctx, wl_adj_rigid):
+    hs = _upload(ctx, wl_adj_rigid)
+    a = ctx.adjoint(hs, wl_adj_rigid.dofs)
+    b = ctx.adjoint(hs, wl_adj_rigid.dofs)
+    assert np.array_equal(a, b)
+
+
+def test_torch_harness_finite_differences(ctx, wl_adj_modes):
+    torch = pytest.importorskip("torch")
+    import torch_harness as TH
+    wl = wl_adj_modes
+    hs = _upload(ctx, wl)
+    T = wl.params.T
+    dofs = torch.tensor(wl.dofs[:32], dtype=torch.float64, requires_grad=True)
+    table = torch.zeros((T * T, 3), dtype=torch.float64, requires_grad=True)
+    E = TH.energy(dofs, table, ctx, hs, T)
+    wts = torch.linspace(0.5, 1.5, 32, dtype=torch.float64)
+    (E * wts).sum().backward()
+    g = dofs.grad.numpy()
+    direct = ctx.adjoint(hs, wl.dofs[:32], w=wts.numpy().astype(np.float32))
+    assert np.allclose(g[:, :6], direct[:, 1:7], rtol=1e-6, atol=0)
+    assert np.allclose(g[:, 6:], direct[:, 7:], rtol=1e-6, atol=0)
+    assert table.grad is not None and np.abs(table.grad.numpy()).max() > 0
+    # central differences of the GPU energy in translation (h = 1e-3 A)
+    base = wl.dofs[:32].copy()
+    for col in (0, 1, 2):
+        dp, dm = base.copy(), base.copy()
+        dp[:, col] += 1e-3
+        dm[:, col] -= 1e-3
+        ep = ctx.adjoint(hs, dp)[:, 0].astype(np.float64)
+        em = ctx.adjoint(hs, dm)[:, 0].astype(np.float64)
+        fd = (ep - em) / (dp[:, col].astype(np.float64) - dm[:, col].astype(np.float64))
+        an = direct[:, 1 + col] / wts.numpy()
+        rel = np.abs(fd - an) / (np.abs(an) + np.abs(fd) + 1e-2)
+        assert np.median(rel) < 2e-3
+
+
+def test_adjoint_vs_reference_forces(ctx, O, wl_adj_rigid):
+    """The reference's reported force is the interpolated force channel plus
+    the pair force minus the plateau force, scaled by 0.01 per step while
+    |F|^2 > 1e18 (reducePotForce's clamp); the exact gradient differs from it
+    by construction.  Stated over the poses in contact whose force is not
+    clamped, with a loose bound on the translation part."""
+    wl = wl_adj_rigid
+    hs = _upload(ctx, wl)
+    g = ctx.adjoint(hs, wl.dofs)
+    f = ctx.score_rigid(hs, wl.dofs)
+    dt, F = g[:, 1:4].astype(np.float64), -f[:, 1:4].astype(np.float64)
+    nF, ng = np.linalg.norm(F, axis=1), np.linalg.norm(dt, axis=1)
+    keep = (ng > 1e-3) & (ng < 1e8)
+    assert keep.sum() >= 20
+    rel = np.linalg.norm(dt - F, axis=1)[keep] / nF[keep]
+    print(f"|exact gradient - reference force| / |force| over {keep.sum()} unclamped poses in "
+          f"contact: median {np.median(rel):.3f}, p90 {np.percentile(rel, 90):.3f}")
+    assert np.median(rel) < 0.5
