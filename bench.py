@@ -1,10 +1,13 @@
 """Benchmark: poses/s of energy + DOF gradient for flexible scoring on one MI355X.
 
-The measured path (`value`) is the fused adjoint kernel: energy and the exact
-gradient with respect to the rigid DOFs and all mode amplitudes, DOFs resident
-in HBM.  Reported beside it: the fused forward kernel with the reference's
-force semantics, and the plain per-item path (host buffers, 100 poses per
-call, as the drop-in service issues them).
+The measured path (`value`) is the fused adjoint: energy and the exact
+gradient with respect to the rigid DOFs, all mode amplitudes and the 12x12
+pair-parameter table (rc, ac, emin of every type pair, summed over poses),
+DOFs resident in HBM.  One step = k_adjoint over all poses + the fixed-order
+reduction of the per-workgroup parameter rows.  Reported beside it: the
+adjoint without parameter gradients, the fused forward kernel with the
+reference's force semantics, and the plain per-item path (host buffers, 100
+poses per call, as the drop-in service issues them).
 
 Workload (default): the flexible-scoring configuration — a 3000-bead receptor
 and an 800-bead ligand (seeded synthetic bodies, 12 bead types, 20% charged),
@@ -14,8 +17,10 @@ with DOFs already resident in HBM.
 
 Prints one JSON line with the metric, a roofline object for the dominant
 kernel (algorithmic bytes per launch / launch time from HIP events on the
-launch stream) and the CPU baseline (the oracle restatement on the host cores,
-bounded sample).
+launch stream) and the CPU baseline: the reference's own CPU path (compiled
+from its sources into oracle/_ref) on the host cores, worker threads taking
+items of 100 poses as the reference's server does, over a bounded sample; the
+oracle restatement stands in when the compiled reference is absent.
 """
 import argparse
 import importlib.util
@@ -48,25 +53,30 @@ def log(*a):
 def make(args, W):
     if args.workload == "modes":
         return W.make_workload("flexible scoring 3000/800, 5+5 modes", 20261203, 3000, 800,
-                               args.poses, 5, 5)
+                               args.poses, 5, 5, with_basis=True)
     return W.make_workload("rigid scoring 1000/300", 20261202, 1000, 300, args.poses)
 
 
 def cpu_baseline(wl, args):
-    """Oracle restatement on the host cores over a bounded sample of poses."""
+    """The reference's CPU path (or, without it, the oracle restatement) on the
+    host cores over a bounded sample of the same poses."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as O
     threads = max(1, min(16, os.cpu_count() or 1))
     n = min(args.cpu_poses, wl.dofs.shape[0])
     d = wl.dofs[:n]
-    f = O.score_modes if wl.modes else O.score_rigid
-    f(wl, d[:min(n, 4 * threads)], nthreads=threads)  # warm-up
-    t0 = time.perf_counter()
-    res = f(wl, d, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="poses/s", cores=threads, kind="port",
-                sample=f"{n} poses of the same workload, {threads} threads, "
-                       f"items of 100 poses, {dt:.1f} s"), res
+    if O.ref_available():
+        kind = "reference"
+        res, dt = O.ref_score_mt(wl, d, nthreads=threads)
+    else:
+        kind = "port"
+        f = O.score_modes if wl.modes else O.score_rigid
+        t0 = time.perf_counter()
+        res = f(wl, d, nthreads=threads)
+        dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="poses/s", cores=threads, kind=kind,
+                sample=f"first {n} poses of the same workload (energy + forces, the reference's "
+                       f"gradient), {threads} worker threads, items of 100 poses, {dt:.1f} s"), res
 
 
 def main():
@@ -76,8 +86,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["modes", "rigid"], default="modes")
     ap.add_argument("--poses", type=int, default=100000)
-    ap.add_argument("--cpu-poses", type=int, default=20000)
+    ap.add_argument("--cpu-poses", type=int, default=50000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pgrad", action="store_true",
+                    help="DOF and mode gradients only (no pair-parameter gradient)")
     ap.add_argument("--plain-poses", type=int, default=5000)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="also run the plain copy kernel over this many bytes "
@@ -96,6 +108,10 @@ def main():
     log(f"workload built in {time.perf_counter() - t0:.1f} s: {wl.name}, {wl.dofs.shape[0]} poses")
     ctx = hip.Context(0)
     hs = ctx.upload_workload(wl)
+    pgrad = not args.no_pgrad and wl.rec_inner_basis is not None
+    if pgrad:
+        ctx.upload_basis(hs["grid"], wl)
+    T = wl.params.T
     n = wl.dofs.shape[0]
     out_w = 27 if wl.modes else 7
     d_in = ctx.to_device(wl.dofs)
@@ -103,23 +119,33 @@ def main():
     sh = ctx.stream()  # the stream every launch below is issued on
     score = ctx.score_modes_dev if wl.modes else ctx.score_rigid_dev
     d_grad = ctx.dev_alloc(n * out_w * 4)
+    d_pg = ctx.dev_alloc(T * T * 3 * 4) if pgrad else None
 
     def fwd_step():
         score(hs, d_in.ptr, n, d_out.ptr, sh)
 
     def step():
+        ctx.adjoint_dev(hs, d_in.ptr, wl.modes, n, None, d_grad.ptr,
+                        d_pg.ptr if pgrad else None, sh)
+
+    def dof_step():
         ctx.adjoint_dev(hs, d_in.ptr, wl.modes, n, None, d_grad.ptr, None, sh)
 
-    # forward kernel, reported beside the headline path
-    for _ in range(max(1, args.warmup)):
-        fwd_step()
-    e0, e1 = ctx.event(), ctx.event()
-    ctx.record(e0, sh)
-    for _ in range(max(1, args.steps // 2)):
-        fwd_step()
-    ctx.record(e1, sh)
-    fwd_ms = ctx.elapsed_ms(e0, e1) / max(1, args.steps // 2)
+    def rate(fn, reps):
+        fn()
+        a, b = ctx.event(), ctx.event()
+        ctx.record(a, sh)
+        for _ in range(reps):
+            fn()
+        ctx.record(b, sh)
+        return ctx.elapsed_ms(a, b) / reps
+
+    # forward kernel and the adjoint without parameter gradients, reported beside
+    reps = max(1, args.steps // 2)
+    fwd_ms = rate(fwd_step, reps)
     log(f"forward kernel: {fwd_ms:.3f} ms/step, {n / fwd_ms * 1e3:.4g} poses/s")
+    dof_ms = rate(dof_step, reps)
+    log(f"adjoint, DOF + mode gradients only: {dof_ms:.3f} ms/step, {n / dof_ms * 1e3:.4g} poses/s")
 
     for _ in range(args.warmup):
         step()
@@ -149,7 +175,7 @@ def main():
         a.release()
         bb.release()
 
-    bpp, bpp_sd = traffic.bytes_per_pose(wl, adjoint=True)
+    bpp, bpp_sd = traffic.bytes_per_pose(wl, adjoint=True, pgrad=pgrad)
     achieved = bpp * n / avg_launch
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK, traffic=None,
@@ -176,8 +202,9 @@ def main():
     if not args.no_cpu:
         cpu, ref = cpu_baseline(wl, args)
         gpu = ctx.to_host(d_out, (n, out_w))[:ref.shape[0]]
-        rel = np.abs(gpu - ref) / np.maximum(np.abs(ref), 1e-30)
-        cpu["gpu_vs_cpu_median_rel_err"] = float(np.median(rel[:, 0]))
+        fin = np.isfinite(ref[:, 0]) & np.isfinite(gpu[:, 0])
+        rel = np.abs(gpu[fin, 0] - ref[fin, 0]) / np.maximum(np.abs(ref[fin, 0]), 1e-30)
+        cpu["forward_vs_reference_energy_median_rel_err"] = float(np.median(rel))
         log(f"cpu baseline: {cpu['value']:.4g} poses/s on {cpu['cores']} threads")
 
     line = {
@@ -197,7 +224,10 @@ def main():
             "workload": wl.name,
             "n_rec": wl.rec.n, "n_lig": wl.lig.n, "modes_rec": wl.rec.M, "modes_lig": wl.lig.M,
             "poses": n,
-            "path": "fused adjoint kernel (energy + exact DOF/mode gradient), device-resident",
+            "path": ("fused adjoint (energy + exact DOF, mode" +
+                     (" and 12x12 pair-parameter" if pgrad else "") +
+                     " gradients), device-resident"),
+            "adjoint_dof_modes_only_poses_per_s": n / dof_ms * 1e3,
             "forward_kernel_poses_per_s": n / fwd_ms * 1e3,
             "plain_item100_poses_per_s": plain,
             "adjoint_vs_plain": value / plain,

@@ -8,6 +8,8 @@ Per bead touched by a pose:
   + 8 * n + 4 * K           neighbour-list descriptor when the bead is in the
                             neighbour-list box, plus 4 bytes per listed neighbour
   + 12 * M                  mode vectors of the bead (flexible scoring)
+  + 64 * T                  the adjoint's parameter gradient: 2T basis values
+                            (S_a, S6_a) at the 8 corners, for beads in a grid
 Per pose: 24 bytes DOF in, 28 bytes result out, plus 4*(M_rec+M_lig) each way
 for mode amplitudes and gradients.
 
@@ -51,12 +53,12 @@ def _deformed(body, amps):
     return x[None] + np.einsum("pm,nmc->pnc", amps.astype(np.float64), modes)
 
 
-def _grid_bytes(inner, outer, p, mtype, charge):
+def _grid_bytes(inner, outer, p, mtype, charge, basis_T=0):
     ing = _inside(inner, p)
     inany = ing | _inside(outer, p)
     g = inany & (mtype[None, :] != 0)
     e = g & (np.abs(charge)[None, :] > 0.001)
-    return 128.0 * (g.sum(axis=1) + e.sum(axis=1))
+    return 128.0 * (g.sum(axis=1) + e.sum(axis=1)) + 64.0 * basis_T * g.sum(axis=1)
 
 
 def _nl_bytes(nl, p, typ):
@@ -70,11 +72,12 @@ def _nl_bytes(nl, p, typ):
     return 8.0 * inb.sum(axis=1) + 4.0 * (K * inb).sum(axis=1)
 
 
-def bytes_per_pose(wl, sample=256, seed=0, adjoint=False):
+def bytes_per_pose(wl, sample=256, seed=0, adjoint=False, pgrad=False):
     """Mean algorithmic bytes per pose over a sample of the workload's poses.
     adjoint=True counts the adjoint's reads: the receptor beads are deformed
     for the neighbour list but not interpolated on the ligand grid (that term
-    is not part of the energy)."""
+    is not part of the energy); pgrad=True adds the basis-grid reads of the
+    pair-parameter gradient."""
     rng = np.random.default_rng(seed)
     n = wl.dofs.shape[0]
     pick = rng.choice(n, size=min(sample, n), replace=False)
@@ -90,7 +93,8 @@ def bytes_per_pose(wl, sample=256, seed=0, adjoint=False):
     xl = _deformed(lig, ampL)
     pl = np.einsum("pij,pnj->pni", R, xl) + t[:, None, :]
     B = 20.0 * lig.n + 12.0 * lig.n * lig.M
-    B = B + _grid_bytes(wl.rec_inner, wl.rec_outer, pl, lig.mtype, lig.charge)
+    B = B + _grid_bytes(wl.rec_inner, wl.rec_outer, pl, lig.mtype, lig.charge,
+                        wl.params.T if pgrad else 0)
     B = B + _nl_bytes(wl.rec_nl, pl, lig.type)
     per_pose_io = 24.0 + 28.0
     if wl.modes:

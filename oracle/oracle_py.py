@@ -147,6 +147,17 @@ def ref_lib():
                                       C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(NLGridC),
                                       C.POINTER(RefTableC), P, C.c_size_t, P, P, P]
+        L.ref_score_rigid_mt.restype = C.c_int
+        L.ref_score_rigid_mt.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                         C.POINTER(IGridC), C.POINTER(NLGridC),
+                                         C.POINTER(RefTableC), P, C.c_size_t, P, C.c_int,
+                                         C.POINTER(C.c_double)]
+        L.ref_score_modes_mt.restype = C.c_int
+        L.ref_score_modes_mt.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                         C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
+                                         C.POINTER(IGridC), C.POINTER(NLGridC),
+                                         C.POINTER(RefTableC), P, C.c_size_t, P, C.c_int,
+                                         C.POINTER(C.c_double)]
         L.ref_read_params.restype = C.c_int
         L.ref_read_params.argtypes = [C.c_uint, C.c_int, P, P, P, P, P, P, P, P]
         L.ref_ratio_lut.restype = C.c_uint
@@ -324,6 +335,31 @@ def ref_score_modes(wl, dofs=None):
                            _p(rec_pot), _p(lig_pot))
     assert rc == 0
     return res, rec_pot, lig_pot
+
+
+def ref_score_mt(wl, dofs=None, nthreads=16):
+    """The reference's CPU path on `nthreads` worker threads taking items of
+    100 poses: (results, seconds of scoring alone)."""
+    L = ref_lib()
+    d = np.ascontiguousarray(wl.dofs if dofs is None else dofs, np.float32)
+    n = d.shape[0]
+    secs = C.c_double(0.0)
+    if wl.modes:
+        res = np.zeros((n, 27), np.float32)
+        rc = L.ref_score_modes_mt(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                                  C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                                  C.byref(nl_c(wl.rec_nl)), C.byref(igrid_c(wl.lig_inner)),
+                                  C.byref(igrid_c(wl.lig_outer)), C.byref(nl_c(wl.lig_nl)),
+                                  C.byref(table_c(wl.params)), _p(d), n, _p(res), nthreads,
+                                  C.byref(secs))
+    else:
+        res = np.zeros((n, 7), np.float32)
+        rc = L.ref_score_rigid_mt(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                                  C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                                  C.byref(nl_c(wl.rec_nl)), C.byref(table_c(wl.params)), _p(d), n,
+                                  _p(res), nthreads, C.byref(secs))
+    assert rc == 0
+    return res, secs.value
 
 
 def ref_read_params(p):

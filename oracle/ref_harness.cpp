@@ -15,14 +15,20 @@
 //                   with n_rec != n_lig)
 //  param table    : fileIO/readFile.cpp:439-527 on a written attract.par
 //  plateau ratios : model/NLGrid.tpp:44-51
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <type_traits>
+#include <thread>
 #include <unistd.h>
+#include <vector>
 
 #include "CPUEnergyService6D.h"
 #include "DataManager.h"
@@ -260,19 +266,27 @@ int ref_beads_rigid(const BodyIn* rec, const BodyIn* lig, const GridIn* inner, c
 
 // Modes scoring: model functions in the order of CPUEnergyService6DModes.tpp:121-327,
 // buffers sized by their own protein.  res = 27 floats per pose.
-int ref_score_modes(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner,
-                    const GridIn* recOuter, const NLIn* recNL, const GridIn* ligInner,
-                    const GridIn* ligOuter, const NLIn* ligNL, const TableIn* tab, const float* dof,
-                    size_t n, float* res, float* recPotOut, float* ligPotOut) {
-    auto pr = make_protein(*rec);
-    auto pl = make_protein(*lig);
-    GridUnion<float> gridRec(make_igrid(*recInner), make_igrid(*recOuter), make_nlgrid(*recNL));
-    GridUnion<float> gridLig(make_igrid(*ligInner), make_igrid(*ligOuter), make_nlgrid(*ligNL));
-    auto pt = make_table(*tab);
-    auto sp = make_sim(*tab);
+struct ModesSetup {
+    std::shared_ptr<Protein<float>> pr, pl;
+    GridUnion<float> gridRec, gridLig;
+    std::shared_ptr<ParamTable<float>> pt;
+    std::shared_ptr<SimParam<float>> sp;
+    ModesSetup(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner, const GridIn* recOuter,
+               const NLIn* recNL, const GridIn* ligInner, const GridIn* ligOuter, const NLIn* ligNL,
+               const TableIn* tab)
+        : pr(make_protein(*rec)), pl(make_protein(*lig)),
+          gridRec(make_igrid(*recInner), make_igrid(*recOuter), make_nlgrid(*recNL)),
+          gridLig(make_igrid(*ligInner), make_igrid(*ligOuter), make_nlgrid(*ligNL)),
+          pt(make_table(*tab)), sp(make_sim(*tab)) {}
+};
+
+static void score_modes_range(const ModesSetup& S, const float* dof, size_t lo, size_t hi, float* res,
+                              float* recPotOut, float* ligPotOut) {
+    Protein<float>* pr = S.pr.get();
+    Protein<float>* pl = S.pl.get();
     const unsigned NR = pr->numAtoms(), NL = pl->numAtoms();
     std::vector<float> trafoRec(3 * NR), defoRec(3 * NR), potRec(4 * NR), trafoLig(3 * NL), potLig(4 * NL);
-    for (size_t i = 0; i < n; ++i) {
+    for (size_t i = lo; i < hi; ++i) {
         DOF_6D_Modes<float> d;
         const float* s = dof + 26 * i;
         d._6D.pos = Vec3<float>(s[0], s[1], s[2]);
@@ -286,12 +300,12 @@ int ref_score_modes(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner
         rotate_translate(pl->xPos(), pl->yPos(), pl->zPos(), d._6D.pos, d._6D.ang, NL, pl->numModes(),
                          d.modesLig, pl->xModes(), pl->yModes(), pl->zModes(), &trafoLig[0],
                          &trafoLig[NL], &trafoLig[2 * NL]);
-        potForce(gridLig.inner.get(), gridLig.outer.get(), pr.get(), &trafoRec[0], &trafoRec[NR],
+        potForce(S.gridLig.inner.get(), S.gridLig.outer.get(), pr, &trafoRec[0], &trafoRec[NR],
                  &trafoRec[2 * NR], &potRec[0], &potRec[NR], &potRec[2 * NR], &potRec[3 * NR]);
         rotate_forces(inv._6D.ang.inv(), NR, &potRec[0], &potRec[NR], &potRec[2 * NR]);
-        potForce(gridRec.inner.get(), gridRec.outer.get(), pl.get(), &trafoLig[0], &trafoLig[NL],
+        potForce(S.gridRec.inner.get(), S.gridRec.outer.get(), pl, &trafoLig[0], &trafoLig[NL],
                  &trafoLig[2 * NL], &potLig[0], &potLig[NL], &potLig[2 * NL], &potLig[3 * NL]);
-        NLPotForce(gridRec.NL.get(), pr.get(), pl.get(), sp.get(), pt.get(), &defoRec[0], &defoRec[NR],
+        NLPotForce(S.gridRec.NL.get(), pr, pl, S.sp.get(), S.pt.get(), &defoRec[0], &defoRec[NR],
                    &defoRec[2 * NR], &trafoLig[0], &trafoLig[NL], &trafoLig[2 * NL], &potLig[0],
                    &potLig[NL], &potLig[2 * NL], &potLig[3 * NL], &potRec[0], &potRec[NR], &potRec[2 * NR]);
         PotForce_Modes<float> red = reducePotForce<float, PotForce_Modes<float>>(
@@ -315,7 +329,92 @@ int ref_score_modes(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner
         if (recPotOut && i == 0) std::copy(potRec.begin(), potRec.begin() + 3 * NR, recPotOut);
         if (ligPotOut && i == 0) std::copy(potLig.begin(), potLig.end(), ligPotOut);
     }
+}
+
+// Worker threads taking items of 100 poses from a shared counter, as the
+// reference's server feeds its workers (server/Server.h:39).  Returns the
+// wall time of the scoring alone (set-up excluded) in *seconds.
+typedef std::function<bool(size_t&, size_t&)> Take;
+
+static void run_items(size_t n, int nthreads, const std::function<void(int, const Take&)>& fn,
+                      double* seconds) {
+    std::atomic<size_t> next{0};
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> th;
+    for (int t = 0; t < std::max(1, nthreads); ++t)
+        th.emplace_back([&, t] {
+            Take take = [&](size_t& lo, size_t& hi) {
+                lo = next.fetch_add(100);
+                if (lo >= n) return false;
+                hi = std::min(n, lo + 100);
+                return true;
+            };
+            fn(t, take);
+        });
+    for (auto& x : th) x.join();
+    if (seconds)
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+int ref_score_modes(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner,
+                    const GridIn* recOuter, const NLIn* recNL, const GridIn* ligInner,
+                    const GridIn* ligOuter, const NLIn* ligNL, const TableIn* tab, const float* dof,
+                    size_t n, float* res, float* recPotOut, float* ligPotOut) {
+    ModesSetup S(rec, lig, recInner, recOuter, recNL, ligInner, ligOuter, ligNL, tab);
+    score_modes_range(S, dof, 0, n, res, recPotOut, ligPotOut);
     return 0;
+}
+
+int ref_score_modes_mt(const BodyIn* rec, const BodyIn* lig, const GridIn* recInner,
+                       const GridIn* recOuter, const NLIn* recNL, const GridIn* ligInner,
+                       const GridIn* ligOuter, const NLIn* ligNL, const TableIn* tab,
+                       const float* dof, size_t n, float* res, int nthreads, double* seconds) {
+    ModesSetup S(rec, lig, recInner, recOuter, recNL, ligInner, ligOuter, ligNL, tab);
+    run_items(n, nthreads, [&](int, const Take& take) {
+        size_t lo, hi;
+        while (take(lo, hi)) score_modes_range(S, dof, lo, hi, res, nullptr, nullptr);
+    }, seconds);
+    return 0;
+}
+
+// Rigid scoring with the reference's CPUEnergyService6D, one item processor
+// per worker thread (server/Server.tpp creates one per worker).
+int ref_score_rigid_mt(const BodyIn* rec, const BodyIn* lig, const GridIn* inner, const GridIn* outer,
+                       const NLIn* nl, const TableIn* tab, const float* dof, size_t n, float* res,
+                       int nthreads, double* seconds) {
+    auto dm = std::make_shared<DataManager>();
+    auto gu = std::make_shared<GridUnion<float>>(make_igrid(*inner), make_igrid(*outer), make_nlgrid(*nl));
+    Common common;
+    common.gridId = dm->add(gu);
+    common.ligId = dm->add(make_protein(*lig));
+    common.recId = dm->add(make_protein(*rec));
+    common.tableId = dm->add(make_table(*tab));
+    common.paramsId = dm->add(make_sim(*tab));
+    CPUEnergyService6D<float> service(dm);
+    std::vector<DOF_6D<float>> in(n);
+    std::vector<Result_6D<float>> out(n);
+    for (size_t i = 0; i < n; ++i) {
+        in[i].pos = Vec3<float>(dof[6 * i], dof[6 * i + 1], dof[6 * i + 2]);
+        in[i].ang = Vec3<float>(dof[6 * i + 3], dof[6 * i + 4], dof[6 * i + 5]);
+    }
+    std::vector<decltype(service.createItemProcessor())> procs;
+    for (int t = 0; t < std::max(1, nthreads); ++t) procs.push_back(service.createItemProcessor());
+    std::atomic<int> bad{0};
+    run_items(n, nthreads, [&](int t, const Take& take) {
+        size_t lo, hi;
+        while (take(lo, hi)) {
+            WorkItem<DOF_6D<float>, Common, Result_6D<float>> item(in.data() + lo, (unsigned)(hi - lo),
+                                                                  &common, out.data() + lo);
+            procs[t](&item);
+            if (!item.isProcessed()) bad = 1;
+        }
+    }, seconds);
+    for (size_t i = 0; i < n; ++i) {
+        res[7 * i] = out[i].E;
+        res[7 * i + 1] = out[i].pos.x; res[7 * i + 2] = out[i].pos.y; res[7 * i + 3] = out[i].pos.z;
+        res[7 * i + 4] = out[i].ang.x; res[7 * i + 5] = out[i].ang.y; res[7 * i + 6] = out[i].ang.z;
+    }
+    return bad ? -1 : 0;
 }
 
 }  // extern "C"
