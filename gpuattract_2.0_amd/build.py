@@ -66,20 +66,25 @@ def build_hip():
 
 
 def build_service():
+    """The C++ drop-in layer (csrc/service -> lib/libattract_service.so) and its
+    test programs (tests/cpp -> tests/cpp/_bin)."""
     d = os.path.join(CSRC, "service")
     srcs = sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".cpp"))
-    if not srcs:
-        return None
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libattract_service.so")
-    lib_srcs = [s for s in srcs if not os.path.basename(s).startswith("test_")]
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INC, "-I", d]
-         + lib_srcs + ["-o", out, "-L", LIB, "-lattract_hip", "-Wl,-rpath,$ORIGIN", "-pthread"])
-    tests = [s for s in srcs if os.path.basename(s).startswith("test_")]
-    for t in tests:
-        exe = os.path.join(LIB, os.path.splitext(os.path.basename(t))[0])
-        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, "-I", d, t, "-o", exe,
-              "-L", LIB, "-lattract_service", "-lattract_hip", "-Wl,-rpath,$ORIGIN", "-pthread"])
+    cxx = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-I", INC, "-pthread"]
+    _run(cxx + ["-shared"] + srcs + ["-o", out, "-L", LIB, "-lattract_hip", "-Wl,-rpath,$ORIGIN"])
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    if os.path.isdir(tdir):
+        bin_dir = os.path.join(tdir, "_bin")
+        os.makedirs(bin_dir, exist_ok=True)
+        # relative run path: the tree is relocated on other machines
+        link = ["-L", LIB, "-lattract_service", "-lattract_hip",
+                "-Wl,-rpath,$ORIGIN/../../../gpuattract_2.0_amd/lib"]
+        _run(cxx + [os.path.join(tdir, "server_semantics.cpp"), "-o",
+                    os.path.join(bin_dir, "server_semantics")])
+        _run(cxx + ["-shared", os.path.join(tdir, "service_driver.cpp"), "-o",
+                    os.path.join(bin_dir, "libservice_driver.so")] + link)
     return out
 
 

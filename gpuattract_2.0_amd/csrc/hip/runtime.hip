@@ -826,4 +826,50 @@ att_status att_event_elapsed_ms(att_ctx* c, void* a, void* b, float* ms) {
     return ATT_OK;
 }
 
+att_status att_stream_create(att_ctx* c, void** stream) {
+    if (!c || !stream) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s;
+    ATT_HIP(c, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = (void*)s;
+    return ATT_OK;
+}
+
+att_status att_stream_destroy(att_ctx* c, void* stream) {
+    if (!c || !stream) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipStreamDestroy((hipStream_t)stream));
+    return ATT_OK;
+}
+
+att_status att_memcpy_h2d_async(att_ctx* c, void* d_dst, const void* src, size_t bytes, void* stream) {
+    if (!c || (bytes && (!d_dst || !src))) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice,
+                              stream ? (hipStream_t)stream : c->stream));
+    return ATT_OK;
+}
+
+att_status att_memcpy_d2h_async(att_ctx* c, void* dst, const void* d_src, size_t bytes, void* stream) {
+    if (!c || (bytes && (!dst || !d_src))) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost,
+                              stream ? (hipStream_t)stream : c->stream));
+    return ATT_OK;
+}
+
+att_status att_event_query(att_ctx* c, void* ev) {
+    if (!c || !ev) return fail(c, ATT_ERR_INVALID, "null argument");
+    const hipError_t e = hipEventQuery((hipEvent_t)ev);
+    if (e == hipErrorNotReady) return ATT_NOT_READY;
+    ATT_HIP(c, e);
+    return ATT_OK;
+}
+
+att_status att_event_sync(att_ctx* c, void* ev) {
+    if (!c || !ev) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipEventSynchronize((hipEvent_t)ev));
+    return ATT_OK;
+}
+
 }  // extern "C"

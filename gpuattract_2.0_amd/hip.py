@@ -45,6 +45,32 @@ PAIR_DTYPE = np.dtype([("rc", np.float32), ("ac", np.float32), ("ipon", np.int32
 _lib = None
 
 
+def protein_desc(body):
+    """att_protein_desc over a workload Body's arrays (which must outlive it)."""
+    return ProteinDesc(body.n, body.M, _p(body.xyz), _p(body.type), _p(body.mtype),
+                       _p(body.charge), _p(body.modes) if body.M else None,
+                       _p(body.mode_force) if body.M else None)
+
+
+def grid_desc(g):
+    ge = g.geom
+    return GridDesc(ge.nx, ge.ny, ge.nz, g.ngrids, ge.ox, ge.oy, ge.oz, ge.spacing, _p(g.data))
+
+
+def nl_desc(nl):
+    ge = nl.geom
+    return NLDesc(ge.nx, ge.ny, ge.nz, ge.ox, ge.oy, ge.oz, ge.spacing, nl.plateau,
+                  _p(nl.desc), _p(nl.list) if len(nl.list) else None, len(nl.list))
+
+
+def param_table(p):
+    """The T*T att_pair_params table of a Params object."""
+    tab = np.zeros(p.T * p.T, PAIR_DTYPE)
+    tab["rc"], tab["ac"], tab["ipon"] = p.rc, p.ac, p.ipon
+    tab["rmin2"], tab["emin"] = p.rmin2, p.emin
+    return tab
+
+
 class AttError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -148,34 +174,21 @@ class Context:
 
     # -- uploads ---------------------------------------------------------
     def upload_protein(self, body):
-        d = ProteinDesc(body.n, body.M, _p(body.xyz), _p(body.type), _p(body.mtype),
-                        _p(body.charge), _p(body.modes) if body.M else None,
-                        _p(body.mode_force) if body.M else None)
+        d = protein_desc(body)
         out = H()
         self._chk(self.L.att_upload_protein(self.h, C.byref(d), C.byref(out)))
         return out.value
 
-    @staticmethod
-    def _gdesc(g):
-        ge = g.geom
-        return GridDesc(ge.nx, ge.ny, ge.nz, g.ngrids, ge.ox, ge.oy, ge.oz, ge.spacing, _p(g.data))
-
     def upload_grid(self, inner, outer, nl=None):
-        gi, go = self._gdesc(inner), self._gdesc(outer)
-        nd = None
-        if nl is not None:
-            ge = nl.geom
-            nd = NLDesc(ge.nx, ge.ny, ge.nz, ge.ox, ge.oy, ge.oz, ge.spacing, nl.plateau,
-                        _p(nl.desc), _p(nl.list) if len(nl.list) else None, len(nl.list))
+        gi, go = grid_desc(inner), grid_desc(outer)
+        nd = nl_desc(nl) if nl is not None else None
         out = H()
         self._chk(self.L.att_upload_grid(self.h, C.byref(gi), C.byref(go),
                                          C.byref(nd) if nd is not None else None, C.byref(out)))
         return out.value
 
     def upload_params(self, p):
-        tab = np.zeros(p.T * p.T, PAIR_DTYPE)
-        tab["rc"], tab["ac"], tab["ipon"] = p.rc, p.ac, p.ipon
-        tab["rmin2"], tab["emin"] = p.rmin2, p.emin
+        tab = param_table(p)
         out = H()
         self._chk(self.L.att_upload_params(self.h, _p(tab), p.T, p.shape, p.dielec, p.epsilon,
                                            p.ffelec, C.byref(out)))

@@ -33,7 +33,8 @@ enum {
     ATT_ERR_HIP = 2,         /* HIP runtime failure (the reference's CUDA_CHECK, commons/macros.h:62-72) */
     ATT_ERR_NOMEM = 3,
     ATT_ERR_HANDLE = 4,      /* unknown or wrong-kind handle (DataManager::get, server/DataManager.cpp:42-49) */
-    ATT_ERR_UNSUPPORTED = 5  /* input outside what the kernels support (sizes, types) */
+    ATT_ERR_UNSUPPORTED = 5, /* input outside what the kernels support (sizes, types) */
+    ATT_NOT_READY = 6        /* att_event_query: the work before the event is still running */
 };
 
 #define ATT_MODES_MAX 10 /* MODES_MAX_NUMBER, model/Types_6D_Config.h:11 */
@@ -199,6 +200,20 @@ att_status att_event_create(att_ctx* ctx, void** ev);
 att_status att_event_destroy(att_ctx* ctx, void* ev);
 att_status att_event_record(att_ctx* ctx, void* ev, void* stream);
 att_status att_event_elapsed_ms(att_ctx* ctx, void* ev_start, void* ev_stop, float* ms);
+
+/* --- streams, asynchronous copies, completion ------------------------------
+ * The pipeline of GPUEnergyService6D::Private (:101-114, :163-432: its own
+ * streams, asynchronous copies from pinned buffers, events polled by the
+ * worker) expressed over the context's device.  Streams are created at the
+ * default priority.  The async copies return once queued; `src`/`dst` must be
+ * pinned (att_host_alloc) for the copy to overlap.  att_event_query returns
+ * ATT_OK when the work before the event has finished, ATT_NOT_READY when not. */
+att_status att_stream_create(att_ctx* ctx, void** stream);
+att_status att_stream_destroy(att_ctx* ctx, void* stream);
+att_status att_memcpy_h2d_async(att_ctx* ctx, void* d_dst, const void* src, size_t bytes, void* stream);
+att_status att_memcpy_d2h_async(att_ctx* ctx, void* dst, const void* d_src, size_t bytes, void* stream);
+att_status att_event_query(att_ctx* ctx, void* ev);
+att_status att_event_sync(att_ctx* ctx, void* ev);
 
 /* Library build identification (arch, flags). */
 const char* att_build_info(void);
