@@ -54,16 +54,22 @@ __device__ __forceinline__ void trilin_grad(const DGrid& g, float px, float py, 
 }
 
 struct PGrad {
-    float* s;      // LDS accumulator, T*T*3, or nullptr when not requested
+    float* s;                     // LDS accumulator, T*T*3, or nullptr when not requested
+    const att_pair_params* tab;   // the LDS copy of the pair table
     unsigned T;
-    float w;       // upstream weight of the current pose
+    float w;                      // upstream weight of the current pose
 };
+
+// Basis channels read per step of the parameter-gradient loop: the loads of
+// kBasisStep channels are issued together (64 B each), so a bead waits for
+// ceil(T / kBasisStep) memory round trips instead of T.
+constexpr unsigned kBasisStep = 4;
 
 // Grid term of one ligand bead on one grid: energy (the forward's per-bead
 // value), gradient, and the basis-grid parameter derivatives.
-__device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, const DPar& par,
-                                              float px, float py, float pz, unsigned mtype,
-                                              float charge, float3& G, const PGrad& pg) {
+__device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, float px, float py,
+                                              float pz, unsigned mtype, float charge, float3& G,
+                                              const PGrad& pg) {
     const unsigned ix = grid_index(px, g.lox, g.dVox_inv, g.nx);
     const unsigned iy = grid_index(py, g.loy, g.dVox_inv, g.ny);
     const unsigned iz = grid_index(pz, g.loz, g.dVox_inv, g.nz);
@@ -89,17 +95,29 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, c
         const float w4 = ax * by * az, w5 = bx * ay * az, w6 = ax * ay * bz, w7 = ax * ay * az;
         const size_t o = ((size_t)iz * (g.ny - 1) + iy) * (g.nx - 1) + ix;
         const float4* row = b.oct + o * (size_t)b.nb * 2;
-        for (unsigned a = 0; a < pg.T; ++a) {
-            const float4 s0 = row[4 * a], s1 = row[4 * a + 1];      // S_a at the 8 corners
-            const float4 t0 = row[4 * a + 2], t1 = row[4 * a + 3];  // S6_a
-            const float S = (s0.x * w0 + s0.y * w1 + s0.z * w2 + s0.w * w3 + s1.x * w4 +
-                             s1.y * w5 + s1.z * w6 + s1.w * w7) * g.vol_inv;
-            const float S6 = (t0.x * w0 + t0.y * w1 + t0.z * w2 + t0.w * w3 + t1.x * w4 +
-                              t1.y * w5 + t1.z * w6 + t1.w * w7) * g.vol_inv;
-            const unsigned k = a * pg.T + (mtype - 1);
-            const float iv = (float)par.tab[k].ipon * pg.w;
-            atomicAdd(&pg.s[3 * k], iv * S);
-            atomicAdd(&pg.s[3 * k + 1], -iv * S6);
+        for (unsigned a0 = 0; a0 < pg.T; a0 += kBasisStep) {
+            float4 v[4 * kBasisStep];
+#pragma unroll
+            for (unsigned j = 0; j < kBasisStep; ++j) {
+                const unsigned a = a0 + j < pg.T ? a0 + j : a0;  // past T: re-read a valid row
+                v[4 * j] = row[4 * a];          // S_a at the 8 corners
+                v[4 * j + 1] = row[4 * a + 1];
+                v[4 * j + 2] = row[4 * a + 2];  // S6_a
+                v[4 * j + 3] = row[4 * a + 3];
+            }
+#pragma unroll
+            for (unsigned j = 0; j < kBasisStep; ++j) {
+                if (a0 + j >= pg.T) break;
+                const float4 s0 = v[4 * j], s1 = v[4 * j + 1], t0 = v[4 * j + 2], t1 = v[4 * j + 3];
+                const float S = (s0.x * w0 + s0.y * w1 + s0.z * w2 + s0.w * w3 + s1.x * w4 +
+                                 s1.y * w5 + s1.z * w6 + s1.w * w7) * g.vol_inv;
+                const float S6 = (t0.x * w0 + t0.y * w1 + t0.z * w2 + t0.w * w3 + t1.x * w4 +
+                                  t1.y * w5 + t1.z * w6 + t1.w * w7) * g.vol_inv;
+                const unsigned k = (a0 + j) * pg.T + (mtype - 1);
+                const float iv = (float)pg.tab[k].ipon * pg.w;
+                atomicAdd(&pg.s[3 * k], iv * S);
+                atomicAdd(&pg.s[3 * k + 1], -iv * S6);
+            }
         }
     }
     return pot.w;
@@ -114,9 +132,9 @@ __device__ __forceinline__ float grid_adj(const DGrid& inner, const DGrid& outer
     G = make_float3(0.f, 0.f, 0.f);
     if (mtype == 0) return 0.f;
     if (!grid_out(inner, px, py, pz))
-        return grid_adj_one(inner, bin, par, px, py, pz, mtype, charge, G, pg);
+        return grid_adj_one(inner, bin, px, py, pz, mtype, charge, G, pg);
     if (!grid_out(outer, px, py, pz))
-        return grid_adj_one(outer, bout, par, px, py, pz, mtype, charge, G, pg);
+        return grid_adj_one(outer, bout, px, py, pz, mtype, charge, G, pg);
     return 0.f;
 }
 
@@ -131,102 +149,97 @@ __device__ __forceinline__ void lj_dparam(float dr2, float dr2_inv, const att_pa
     dem = dr2 < p.rmin2 ? (float)p.ipon - 1.f : 0.f;
 }
 
-// Neighbour-list term of one ligand bead for the adjoint: energy (the
-// forward's per-bead value), G -= pair force, receptor(j, F) gets the force
-// on the receptor side, parameter derivatives into pg.
-template <class RecPos, class RecType, class RecCharge, class Reaction>
-__device__ __forceinline__ float nl_adj(const DNL& nl, const DPar& par, const att_pair_params* tab,
-                                        RecPos recPos, RecType recType, RecCharge recCharge,
-                                        Reaction reaction, float px, float py, float pz,
-                                        unsigned ligType, float ligCharge, float3& G,
-                                        const PGrad& pg) {
-    if (ligType == 0) return 0.f;
+// Neighbour-list cell of a ligand bead: {count, start} of its list, {0, 0}
+// when the bead has no neighbour-list term (same bounds and index as nl_bead).
+__device__ __forceinline__ uint2 nl_cell(const DNL& nl, float px, float py, float pz,
+                                         unsigned ligType) {
+    if (ligType == 0) return make_uint2(0u, 0u);
     if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
           (pz >= nl.loz && pz <= nl.hiz)))
-        return 0.f;
+        return make_uint2(0u, 0u);
     const int ix = (int)roundf((px - nl.lox) * nl.dVox_inv);
     const int iy = (int)roundf((py - nl.loy) * nl.dVox_inv);
     const int iz = (int)roundf((pz - nl.loz) * nl.dVox_inv);
-    const uint2 d = nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
-    float e = 0.f;
-    const unsigned lt1 = ligType - 1;
-    const bool ligCharged = ligCharge != 0.f;
-    const uint32_t* lst = nl.list + d.y;
-    float drcP = 0.f, dacP = 0.f, demP = 0.f;
-    for (unsigned j0 = 0; j0 < d.x; j0 += 4) {
-        unsigned idx[4];
-        float3 rp[4];
-        unsigned rt[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) idx[k] = lst[j0 + k < d.x ? j0 + k : j0];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { rp[k] = recPos(idx[k]); rt[k] = recType(idx[k]); }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (j0 + k >= d.x) break;
-            const unsigned nIdx = idx[k];
-            const float ddx = px - rp[k].x, ddy = py - rp[k].y, ddz = pz - rp[k].z;
-            const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (!(dr2 <= nl.P2)) continue;
-            if (rt[k] - 1u >= par.T) continue;
-            const float dr2_inv = (float)(1.0 / (double)dr2);
-            const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
-            const unsigned kk = par.T * (rt[k] - 1) + lt1;
-            const att_pair_params p = tab[kk];
-            float vx, vy, vz, ve, Fx, Fy, Fz;
-            lj(dr2, dr2_inv, dx, dy, dz, p, par.shape, Fx, Fy, Fz, ve);
-            e += ve;
-            float pe;
-            lj(nl.P2, nl.P2_inv, 0.f, 0.f, 0.f, p, par.shape, vx, vy, vz, pe);
-            e -= pe;
-            if (ligCharged) {
-                const float qlr = ligCharge * recCharge(nIdx) * par.ffelec;
-                if (fabsf(qlr) > 0.001f) {
-                    coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
-                    e += ve;
-                    Fx += vx; Fy += vy; Fz += vz;
-                    coulomb(nl.P2_inv, 0.f, 0.f, 0.f, qlr, par.dielec, vx, vy, vz, ve);
-                    e -= ve;
-                }
-            }
-            G.x -= Fx; G.y -= Fy; G.z -= Fz;
-            reaction(nIdx, Fx, Fy, Fz);
-            if (pg.s) {
-                float drc, dac, dem;
-                lj_dparam(dr2, dr2_inv, p, par.shape, drc, dac, dem);
-                lj_dparam(nl.P2, nl.P2_inv, p, par.shape, drcP, dacP, demP);
-                atomicAdd(&pg.s[3 * kk], pg.w * (drc - drcP));
-                atomicAdd(&pg.s[3 * kk + 1], pg.w * (dac - dacP));
-                if (dem != demP) atomicAdd(&pg.s[3 * kk + 2], pg.w * (dem - demP));
-            }
-        }
-    }
-    return e;
+    return nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
 }
 
+// Exclusive prefix sum of one value per thread over the workgroup (at most
+// kAdjWaves waves); `total` receives the sum.  Contains two barriers.
+__device__ __forceinline__ unsigned block_scan(unsigned v, unsigned* s_w, unsigned& total) {
+    const unsigned lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    unsigned x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned y = __shfl_up(x, d);
+        if (lane >= (unsigned)d) x += y;
+    }
+    if (lane == kWave - 1) s_w[wave] = x;
+    __syncthreads();
+    unsigned before = 0, tot = 0;
+    const unsigned nw = blockDim.x / kWave;
+    for (unsigned w = 0; w < nw; ++w) {
+        const unsigned t = s_w[w];
+        before += w < wave ? t : 0u;
+        tot += t;
+    }
+    total = tot;
+    __syncthreads();
+    return before + x - v;
+}
+
+constexpr int kAdjWaves = 4;          // 256-thread workgroups
+
+// Byte offset of the per-bead neighbour-list cells behind the pair table.
+__host__ __device__ constexpr size_t adjoint_nl_offset(unsigned tabSlots) {
+    return (sizeof(att_pair_params) * (size_t)tabSlots + 15) & ~(size_t)15;
+}
+constexpr unsigned kSlots = 256;      // queued ligand beads per pair round
+
+// Workgroup-shared state of the pair phase (static LDS).
+struct PairLDS {
+    float4 p[kSlots];         // transformed bead position, charge
+    float4 x[kSlots];         // deformed bead coordinates x~, bead index | type << 16
+    unsigned start[kSlots];   // first list entry
+    unsigned end[kSlots];     // inclusive prefix of the pair counts
+    float4 G[kSlots];         // sum of pair forces on the bead
+    float4 head[kAdjWaves];   // a wave's share of the bead it starts inside
+    unsigned scan[kAdjWaves];
+    unsigned qn;
+};
+
+// k_adjoint: one 256-thread workgroup per pose at a time, persistent.
+//  B1   every ligand bead: transform, grid term with its exact gradient (and
+//       the basis-grid parameter derivatives), chain rule; beads with a
+//       neighbour list are recorded with their {count, start}.
+//  Q    the recorded beads are compacted in bead order (block scan).
+//  B2   pair rounds of up to kSlots beads: the pairs of the round are
+//       flattened (prefix sum of counts) and split into four contiguous
+//       ranges, one per wave, walked in chunks of 64 consecutive pairs: list
+//       reads are coalesced and every lane has work until the range ends.
+//       Receptor beads are deformed on the fly from their padded mode row
+//       (one cache line).  Pair forces are summed per bead by a segmented
+//       wave reduction; a bead split between two waves gets the later wave's
+//       share through `head`, added in wave order, so results are
+//       deterministic.  Then the chain rule for the NL force of each bead.
 template <int MM>
-__global__ __launch_bounds__(256) void k_adjoint(DProt rec, DProt lig, DGrid rInner, DGrid rOuter,
-                                                 DNL nl, DPar par, DBasis bin, DBasis bout,
-                                                 const float* __restrict__ dofs, unsigned dofStride,
-                                                 const float* __restrict__ wts,
-                                                 float* __restrict__ out, unsigned outStride,
-                                                 float* __restrict__ pgPartial, unsigned n,
-                                                 unsigned tabSlots) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
+    DProt rec, DProt lig, DGrid rInner, DGrid rOuter, DNL nl, DPar par, DBasis bin, DBasis bout,
+    const float* __restrict__ dofs, unsigned dofStride, const float* __restrict__ wts,
+    float* __restrict__ out, unsigned outStride, float* __restrict__ pgPartial, unsigned n,
+    unsigned tabSlots) {
     extern __shared__ att_pair_params s_dyn[];
-    __shared__ float s_red[4][13 + 2 * ATT_MODES_MAX];
-    __shared__ unsigned s_qn;
+    __shared__ float s_red[kAdjWaves][13 + 2 * ATT_MODES_MAX];
+    __shared__ PairLDS S;
     const att_pair_params* tab = stage_table(par, s_dyn);
-    float* s_rx = reinterpret_cast<float*>(s_dyn + tabSlots);
-    float* s_ry = s_rx + rec.n;
-    float* s_rz = s_ry + rec.n;
-    unsigned short* s_rt = reinterpret_cast<unsigned short*>(s_rz + rec.n);
-    unsigned* s_queue = reinterpret_cast<unsigned*>(s_rt + ((rec.n + 1) & ~1u));
-    float* s_pg = pgPartial ? reinterpret_cast<float*>(s_queue + pow2_ceil(lig.n)) : nullptr;
+    uint2* s_nl = reinterpret_cast<uint2*>(reinterpret_cast<char*>(s_dyn) +
+                                           adjoint_nl_offset(tabSlots));  // per bead {count, start}
+    unsigned short* s_queue = reinterpret_cast<unsigned short*>(s_nl + lig.n);
+    float* s_pg = pgPartial ? reinterpret_cast<float*>(s_queue + ((lig.n + 1) & ~1u)) : nullptr;
     const unsigned npg = 3 * par.T * par.T;
     if (s_pg)
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    const unsigned wave = threadIdx.x / kWave;
     constexpr int MA = MM > 0 ? MM : 1;
     const float4* __restrict__ rpos = rec.pos;
     const float4* __restrict__ rmodesT = rec.modesT;
@@ -234,11 +247,10 @@ __global__ __launch_bounds__(256) void k_adjoint(DProt rec, DProt lig, DGrid rIn
 
     for (unsigned pose = blockIdx.x; pose < n; pose += gridDim.x) {
         __syncthreads();  // LDS of the previous pose is no longer read
-        if (threadIdx.x == 0) s_qn = 0;
         const float* d = dofs + (size_t)dofStride * pose;
         const float tx = d[0], ty = d[1], tz = d[2];
         const float wp = wts ? wts[pose] : 1.f;
-        const PGrad pg{s_pg, par.T, wp};
+        const PGrad pg{s_pg, tab, par.T, wp};
         float ampR[MA], ampL[MA];
 #pragma unroll
         for (int m = 0; m < MA; ++m) {
@@ -257,17 +269,6 @@ __global__ __launch_bounds__(256) void k_adjoint(DProt rec, DProt lig, DGrid rIn
 #pragma unroll
         for (int m = 0; m < MA; ++m) { gR[m] = 0.f; gL[m] = 0.f; }
 
-        // Phase A: deformed receptor into LDS (the neighbour list's positions).
-        for (unsigned j = threadIdx.x; j < rec.n; j += blockDim.x) {
-            const float4 x = rpos[j];
-            BeadModes<MM> md;
-            load_modes(rec, j, md);
-            const V3 df = deform(md, Mr, x.x, x.y, x.z, ampR);
-            s_rx[j] = df.x; s_ry[j] = df.y; s_rz[j] = df.z;
-            s_rt[j] = (unsigned short)rec.type[j];
-        }
-        __syncthreads();
-
         auto place = [&](unsigned i, float4& x, V3& xt) __attribute__((always_inline)) {
             x = lig.pos[i];
             BeadModes<MM> md;
@@ -278,65 +279,222 @@ __global__ __launch_bounds__(256) void k_adjoint(DProt rec, DProt lig, DGrid rIn
             return p;
         };
         // Chain rule through p = R x~ + t for one bead's dE/dp = G.
-        auto add_grad = [&](unsigned i, const V3& xt, const float3& G) __attribute__((always_inline)) {
-            acc[1] += G.x; acc[2] += G.y; acc[3] += G.z;
-            acc[4] += G.x * xt.x; acc[5] += G.x * xt.y; acc[6] += G.x * xt.z;
-            acc[7] += G.y * xt.x; acc[8] += G.y * xt.y; acc[9] += G.y * xt.z;
-            acc[10] += G.z * xt.x; acc[11] += G.z * xt.y; acc[12] += G.z * xt.z;
+        auto add_grad = [&](unsigned i, float xx, float xy, float xz, float Gx, float Gy, float Gz)
+            __attribute__((always_inline)) {
+            acc[1] += Gx; acc[2] += Gy; acc[3] += Gz;
+            acc[4] += Gx * xx; acc[5] += Gx * xy; acc[6] += Gx * xz;
+            acc[7] += Gy * xx; acc[8] += Gy * xy; acc[9] += Gy * xz;
+            acc[10] += Gz * xx; acc[11] += Gz * xy; acc[12] += Gz * xz;
             if (MM > 0) {
                 // G . R m = (R^T G) . m
-                const float gx = R[0] * G.x + R[3] * G.y + R[6] * G.z;
-                const float gy = R[1] * G.x + R[4] * G.y + R[7] * G.z;
-                const float gz = R[2] * G.x + R[5] * G.y + R[8] * G.z;
+                const float gx = R[0] * Gx + R[3] * Gy + R[6] * Gz;
+                const float gy = R[1] * Gx + R[4] * Gy + R[7] * Gz;
+                const float gz = R[2] * Gx + R[5] * Gy + R[8] * Gz;
                 BeadModes<MM> md;
                 load_modes(lig, i, md);
                 mode_dots(md, Ml, -gx, -gy, -gz, gL);  // mode_dots subtracts
             }
         };
-        // Phase B1: grid term of every ligand bead; beads with neighbour lists queued.
+
+        // B1: grid term of every ligand bead; neighbour-list cells recorded.
         for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
             float4 x;
             V3 xt;
             const V3 p = place(i, x, xt);
             float3 G;
             acc[0] += grid_adj(rInner, rOuter, bin, bout, par, p.x, p.y, p.z, lig.mtype[i], x.w, G, pg);
-            add_grad(i, xt, G);
-            const unsigned cnt = nl_count(nl, p.x, p.y, p.z, lig.type[i]);
-            if (cnt) s_queue[atomicAdd(&s_qn, 1u)] = (cnt < 0xFFFFu ? cnt : 0xFFFFu) << 16 | i;
+            add_grad(i, xt.x, xt.y, xt.z, G.x, G.y, G.z);
+            s_nl[i] = nl_cell(nl, p.x, p.y, p.z, lig.type[i]);
         }
         __syncthreads();
-        const unsigned qn = s_qn;
-        const unsigned qp = pow2_ceil(qn);
-        for (unsigned q = qn + threadIdx.x; q < qp; q += blockDim.x) s_queue[q] = 0;
+
+        // Q: beads with listed neighbours, in bead order.
+        {
+            const unsigned per = (lig.n + blockDim.x - 1) / blockDim.x;
+            const unsigned lo = threadIdx.x * per, hi = min(lig.n, lo + per);
+            unsigned cnt = 0;
+            for (unsigned i = lo; i < hi; ++i) cnt += s_nl[i].x != 0u;
+            unsigned total;
+            unsigned o = block_scan(cnt, S.scan, total);
+            for (unsigned i = lo; i < hi; ++i)
+                if (s_nl[i].x != 0u) s_queue[o++] = (unsigned short)i;
+            if (threadIdx.x == 0) S.qn = total;
+        }
         __syncthreads();
-        bitonic_sort_desc(s_queue, qp);
-        // Phase B2: pair terms, longest lists first.
-        auto recPos = [=](unsigned j) { return make_float3(s_rx[j], s_ry[j], s_rz[j]); };
-        auto recType = [=](unsigned j) { return (unsigned)s_rt[j]; };
-        auto reaction = [&](unsigned j, float fx, float fy, float fz) __attribute__((always_inline)) {
-            if (MM > 0) {
-                const float4* row = rmodesT + (size_t)j * rowT;
-                constexpr unsigned Q = (3 * MA + 3) / 4;
-                float mt[4 * Q];
-#pragma unroll
-                for (unsigned q = 0; q < Q; ++q) {
-                    const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                    mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
-                }
-#pragma unroll
-                for (unsigned m = 0; m < (unsigned)MA; ++m)
-                    if (m < Mr) gR[m] += fx * mt[3 * m] + fy * mt[3 * m + 1] + fz * mt[3 * m + 2];
+        const unsigned qn = S.qn;
+
+        // B2: pair rounds.
+        for (unsigned base = 0; base < qn; base += kSlots) {
+            const unsigned ns = min(kSlots, qn - base);
+            unsigned cnt = 0;
+            if (threadIdx.x < ns) {
+                const unsigned i = s_queue[base + threadIdx.x];
+                float4 x;
+                V3 xt;
+                const V3 p = place(i, x, xt);
+                const uint2 c = s_nl[i];
+                cnt = c.x;
+                S.p[threadIdx.x] = make_float4(p.x, p.y, p.z, x.w);
+                S.x[threadIdx.x] = make_float4(xt.x, xt.y, xt.z, __uint_as_float(i | lig.type[i] << 16));
+                S.start[threadIdx.x] = c.y;
+                S.G[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-        };
-        for (unsigned q = threadIdx.x; q < qn; q += blockDim.x) {
-            const unsigned i = s_queue[q] & 0xFFFFu;
-            float4 x;
-            V3 xt;
-            const V3 p = place(i, x, xt);
-            float3 G = make_float3(0.f, 0.f, 0.f);
-            acc[0] += nl_adj(nl, par, tab, recPos, recType, RecChargeG{rpos}, reaction, p.x, p.y,
-                             p.z, lig.type[i], x.w, G, pg);
-            add_grad(i, xt, G);
+            unsigned P;
+            const unsigned ex = block_scan(cnt, S.scan, P);
+            if (threadIdx.x < ns) S.end[threadIdx.x] = ex + cnt;
+            if (lane == 0) S.head[wave] = make_float4(0.f, 0.f, 0.f, 0.f);
+            __syncthreads();
+
+            // this wave's range of pairs and the slot it starts in
+            const unsigned per = (P + kAdjWaves - 1) / kAdjWaves;
+            const unsigned rs = min(P, wave * per), re = min(P, rs + per);
+            unsigned b0 = 0;
+            {
+                unsigned lo = 0, hi = ns;  // first slot whose end exceeds rs
+                while (lo < hi) {
+                    const unsigned mid = (lo + hi) >> 1;
+                    if (S.end[mid] > rs) hi = mid; else lo = mid + 1;
+                }
+                b0 = lo;
+            }
+            const unsigned firstSlot = b0;
+            const bool headSplit = rs < re && rs > (b0 ? S.end[b0 - 1] : 0u);
+            for (unsigned B = rs; B < re; B += kWave) {
+                const unsigned p = B + lane;
+                const bool valid = p < re;
+                // Slot of every lane: lane j holds the end of slot b0 + j (ends
+                // increase strictly, the first exceeds B); the lane's slot is b0 +
+                // the number of those ends at or below its pair, found by a binary
+                // search across the lanes.
+                const unsigned e = b0 + lane < ns ? S.end[b0 + lane] : 0xFFFFFFFFu;
+                // (a range of 64 candidates takes 7 halvings; every lane takes
+                // part in each shuffle, the source lane clamped to the wave)
+                unsigned lo = 0, hi = kWave;
+#pragma unroll
+                for (int step = 0; step < 7; ++step) {
+                    const unsigned mid = (lo + hi) >> 1;
+                    const unsigned em = __shfl(e, (int)(mid < (unsigned)kWave ? mid : kWave - 1));
+                    if (lo < hi) {
+                        if (em <= p) lo = mid + 1; else hi = mid;
+                    }
+                }
+                const unsigned s = b0 + lo;
+                b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
+
+                float Fx = 0.f, Fy = 0.f, Fz = 0.f;
+                if (valid) {
+                    const float4 bp = S.p[s];
+                    const unsigned meta = __float_as_uint(S.x[s].w);
+                    const unsigned lt1 = (meta >> 16) - 1u;
+                    const unsigned begin = s ? S.end[s - 1] : 0u;
+                    const unsigned j = nl.list[S.start[s] + (p - begin)];
+                    const float4 rx = rpos[j];
+                    const unsigned rt = rec.type[j];
+                    float mt[MM > 0 ? 4 * ((3 * MA + 3) / 4) : 4];
+                    float yx = rx.x, yy = rx.y, yz = rx.z;
+                    if (MM > 0) {
+                        constexpr unsigned Q = (3 * MA + 3) / 4;
+                        const float4* row = rmodesT + (size_t)j * rowT;
+#pragma unroll
+                        for (unsigned q = 0; q < Q; ++q) {
+                            const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                            mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
+                        }
+                        // the deformation of deform(), same order of operations
+#pragma unroll
+                        for (unsigned m = 0; m < (unsigned)MA; ++m)
+                            if (m < Mr) {
+                                const float a = ampR[m];
+                                yx = yx + a * mt[3 * m];
+                                yy = yy + a * mt[3 * m + 1];
+                                yz = yz + a * mt[3 * m + 2];
+                            }
+                    }
+                    const float ddx = bp.x - yx, ddy = bp.y - yy, ddz = bp.z - yz;
+                    const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (dr2 <= nl.P2 && rt - 1u < par.T) {
+                        const float dr2_inv = (float)(1.0 / (double)dr2);
+                        const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
+                        const unsigned kk = par.T * (rt - 1) + lt1;
+                        const att_pair_params pp = tab[kk];
+                        float vx, vy, vz, ve;
+                        lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, Fx, Fy, Fz, ve);
+                        float e2 = ve;
+                        lj(nl.P2, nl.P2_inv, 0.f, 0.f, 0.f, pp, par.shape, vx, vy, vz, ve);
+                        e2 -= ve;
+                        if (bp.w != 0.f) {
+                            const float qlr = bp.w * rx.w * par.ffelec;
+                            if (fabsf(qlr) > 0.001f) {
+                                coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+                                e2 += ve;
+                                Fx += vx; Fy += vy; Fz += vz;
+                                coulomb(nl.P2_inv, 0.f, 0.f, 0.f, qlr, par.dielec, vx, vy, vz, ve);
+                                e2 -= ve;
+                            }
+                        }
+                        acc[0] += e2;
+                        if (MM > 0) {
+#pragma unroll
+                            for (unsigned m = 0; m < (unsigned)MA; ++m)
+                                if (m < Mr) gR[m] += Fx * mt[3 * m] + Fy * mt[3 * m + 1] + Fz * mt[3 * m + 2];
+                        }
+                        if (pg.s) {
+                            float drc, dac, dem, drcP, dacP, demP;
+                            lj_dparam(dr2, dr2_inv, pp, par.shape, drc, dac, dem);
+                            lj_dparam(nl.P2, nl.P2_inv, pp, par.shape, drcP, dacP, demP);
+                            atomicAdd(&pg.s[3 * kk], pg.w * (drc - drcP));
+                            atomicAdd(&pg.s[3 * kk + 1], pg.w * (dac - dacP));
+                            if (dem != demP) atomicAdd(&pg.s[3 * kk + 2], pg.w * (dem - demP));
+                        }
+                    }
+                }
+                // sum of the forces of each run of lanes sharing a slot, in its first lane
+                const unsigned seg = valid ? s : 0xFFFFFFFFu;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const float ox = __shfl_down(Fx, o), oy = __shfl_down(Fy, o), oz = __shfl_down(Fz, o);
+                    const unsigned os = __shfl_down(seg, o);
+                    if (lane + o < (unsigned)kWave && os == seg) { Fx += ox; Fy += oy; Fz += oz; }
+                }
+                const unsigned prev = __shfl_up(seg, 1);
+                if (valid && (lane == 0 || prev != seg)) {
+                    if (headSplit && seg == firstSlot) {
+                        float4 h = S.head[wave];
+                        h.x += Fx; h.y += Fy; h.z += Fz;
+                        S.head[wave] = h;
+                    } else {
+                        float4 g = S.G[seg];
+                        g.x += Fx; g.y += Fy; g.z += Fz;
+                        S.G[seg] = g;
+                    }
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                // shares of beads split between waves, in wave order
+                for (unsigned w = 1; w < (unsigned)kAdjWaves; ++w) {
+                    const unsigned ws = min(P, w * per);
+                    if (ws >= P) break;
+                    unsigned lo = 0, hi = ns;
+                    while (lo < hi) {
+                        const unsigned mid = (lo + hi) >> 1;
+                        if (S.end[mid] > ws) hi = mid; else lo = mid + 1;
+                    }
+                    if (ws > (lo ? S.end[lo - 1] : 0u)) {
+                        float4 g = S.G[lo];
+                        const float4 h = S.head[w];
+                        g.x += h.x; g.y += h.y; g.z += h.z;
+                        S.G[lo] = g;
+                    }
+                }
+            }
+            __syncthreads();
+            // chain rule for the neighbour-list force of each bead of the round
+            if (threadIdx.x < ns) {
+                const float4 g = S.G[threadIdx.x], xs = S.x[threadIdx.x];
+                add_grad(__float_as_uint(xs.w) & 0xFFFFu, xs.x, xs.y, xs.z, -g.x, -g.y, -g.z);
+            }
+            __syncthreads();  // slot arrays are reused by the next round
         }
 
         // Reduction and epilogue.
@@ -356,11 +514,10 @@ __global__ __launch_bounds__(256) void k_adjoint(DProt rec, DProt lig, DGrid rIn
         __syncthreads();
         if (threadIdx.x == 0) {
             float v[13 + 2 * ATT_MODES_MAX];
-            const int nw = blockDim.x / kWave;
 #pragma unroll
             for (int k = 0; k < 13 + 2 * ATT_MODES_MAX; ++k) {
                 float s = s_red[0][k];
-                for (int ww = 1; ww < nw; ++ww) s += s_red[ww][k];
+                for (int ww = 1; ww < kAdjWaves; ++ww) s += s_red[ww][k];
                 v[k] = s;
             }
             float da[3];
@@ -389,8 +546,17 @@ __global__ void k_pgrad_reduce(const float* __restrict__ partial, unsigned rows,
                                float* __restrict__ pg) {
     const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols) return;
+    // rows are added in order; the loads of 16 rows are issued together
     double s = 0.0;
-    for (unsigned r = 0; r < rows; ++r) s += partial[(size_t)r * cols + c];
+    unsigned r = 0;
+    for (; r + 16 <= rows; r += 16) {
+        float v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = partial[(size_t)(r + k) * cols + c];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s += v[k];
+    }
+    for (; r < rows; ++r) s += partial[(size_t)r * cols + c];
     pg[c] = (float)s;
 }
 
@@ -418,9 +584,24 @@ __global__ __launch_bounds__(256) void k_basis_octets(const float* __restrict__ 
 
 size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool pgrad,
                          unsigned* tabSlots) {
-    return modes_lds_bytes(rec, lig, par, tabSlots) +
+    (void)rec;
+    const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
+    if (tabSlots) *tabSlots = slots;
+    return adjoint_nl_offset(slots) + sizeof(uint2) * (size_t)lig.n +
+           sizeof(unsigned short) * (((size_t)lig.n + 1) & ~(size_t)1) +
            (pgrad ? sizeof(float) * 3 * (size_t)par.T * par.T : 0);
 }
+
+unsigned adjoint_blocks_per_cu(unsigned mm, size_t lds) {
+    int b = 0;
+    hipError_t e;
+    if (mm == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adjoint<0>, 256, lds);
+    else if (mm <= 5) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adjoint<5>, 256, lds);
+    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adjoint<ATT_MODES_MAX>, 256, lds);
+    return e == hipSuccess && b > 0 ? (unsigned)b : 1u;
+}
+
+size_t adjoint_static_lds() { return sizeof(PairLDS) + sizeof(float) * kAdjWaves * (13 + 2 * ATT_MODES_MAX); }
 
 hipError_t launch_adjoint(const DProt& rec, const DProt& lig, const DGrid& inner,
                           const DGrid& outer, const DNL& nl, const DPar& par, const DBasis& bin,

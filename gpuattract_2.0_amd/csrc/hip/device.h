@@ -310,12 +310,22 @@ __device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
     const unsigned lt1 = ligType - 1;
     const bool ligCharged = ligCharge != 0.f;  // else q_lig*q_rec*ffelec = 0: no Coulomb term
     const uint32_t* lst = nl.list + d.y;
+    // Indices of the next batch are loaded while the current batch computes.
+    unsigned nidx[4];
+    if (d.x != 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nidx[k] = lst[(unsigned)k < d.x ? k : 0];
+    }
     for (unsigned j0 = 0; j0 < d.x; j0 += 4) {
         unsigned idx[4];
         float3 rp[4];
         unsigned rt[4];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) idx[k] = lst[j0 + k < d.x ? j0 + k : j0];
+        for (int k = 0; k < 4; ++k) idx[k] = nidx[k];
+        if (j0 + 4 < d.x) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) nidx[k] = lst[j0 + 4 + k < d.x ? j0 + 4 + k : j0 + 4];
+        }
 #pragma unroll
         for (int k = 0; k < 4; ++k) { rp[k] = recPos(idx[k]); rt[k] = recType(idx[k]); }
         // Distances of the four pairs first, then their plateau ratios, so the

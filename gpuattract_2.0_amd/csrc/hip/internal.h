@@ -38,6 +38,10 @@ hipError_t launch_octets(const float4* grid, float4* oct, unsigned nx, unsigned 
 
 size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool pgrad,
                          unsigned* tabSlots);
+// Static LDS of k_adjoint (pair-phase slots, reductions), and how many of its
+// 256-thread workgroups one CU holds with `lds` bytes of dynamic LDS.
+size_t adjoint_static_lds();
+unsigned adjoint_blocks_per_cu(unsigned mm, size_t lds);
 
 hipError_t launch_adjoint(const DProt& rec, const DProt& lig, const DGrid& inner,
                           const DGrid& outer, const DNL& nl, const DPar& par, const DBasis& bin,

@@ -588,13 +588,15 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     const bool pgrad = d_pgrad != nullptr;
     if (pgrad && (!g->bin.oct || g->bin.nb != 2 * p->d.T))
         return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
-    const size_t lds = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, nullptr) +
-                       sizeof(float) * 4 * 33 + 16;
-    if (lds > c->maxLdsPerBlock)
-        return fail(c, ATT_ERR_UNSUPPORTED, "receptor too large for the LDS-staged adjoint kernel");
+    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, nullptr);
+    if (dyn + adjoint_static_lds() > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const unsigned cols = 3 * p->d.T * p->d.T;
-    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, (size_t)c->numCU * 3));
+    // persistent grid: as many workgroups as the device holds at once
+    const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
+    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, dyn);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
     if (pgrad && (st = grow(c, &c->d_pg, &c->d_pg_cap, sizeof(float) * (size_t)blocks * cols)) != ATT_OK)
         return st;
     if (n == 0) {
