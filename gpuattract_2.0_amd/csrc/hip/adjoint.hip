@@ -187,33 +187,29 @@ __device__ __forceinline__ unsigned block_scan(unsigned v, unsigned* s_w, unsign
     return before + x - v;
 }
 
-constexpr int kAdjWaves = 4;          // 256-thread workgroups
-
-// Byte offset of the per-bead neighbour-list cells behind the pair table.
-__host__ __device__ constexpr size_t adjoint_nl_offset(unsigned tabSlots) {
-    return (sizeof(att_pair_params) * (size_t)tabSlots + 15) & ~(size_t)15;
-}
-constexpr unsigned kSlots = 256;      // queued ligand beads per pair round
-
-// Workgroup-shared state of the pair phase (static LDS).
+// Workgroup-shared state of one pose (static LDS); WAVES waves per pose,
+// 64 * WAVES queued ligand beads per pair round.
+template <int WAVES>
 struct PairLDS {
+    static constexpr unsigned kSlots = 64u * WAVES;
     float4 p[kSlots];         // transformed bead position, charge
     float4 x[kSlots];         // deformed bead coordinates x~, bead index | type << 16
     unsigned start[kSlots];   // first list entry
     unsigned end[kSlots];     // inclusive prefix of the pair counts
     float4 G[kSlots];         // sum of pair forces on the bead
-    float4 head[kAdjWaves];   // a wave's share of the bead it starts inside
-    unsigned scan[kAdjWaves];
+    float4 head[WAVES];       // a wave's share of the bead it starts inside
+    unsigned scan[WAVES];
     unsigned qn;
+    float red[WAVES][13 + 2 * ATT_MODES_MAX];
 };
 
-// k_adjoint: one 256-thread workgroup per pose at a time, persistent.
+// Adjoint of one pose by one workgroup of WAVES waves.
 //  B1   every ligand bead: transform, grid term with its exact gradient (and
 //       the basis-grid parameter derivatives), chain rule; beads with a
 //       neighbour list are recorded with their {count, start}.
 //  Q    the recorded beads are compacted in bead order (block scan).
 //  B2   pair rounds of up to kSlots beads: the pairs of the round are
-//       flattened (prefix sum of counts) and split into four contiguous
+//       flattened (prefix sum of counts) and split into WAVES contiguous
 //       ranges, one per wave, walked in chunks of 64 consecutive pairs: list
 //       reads are coalesced and every lane has work until the range ends.
 //       Receptor beads are deformed on the fly from their padded mode row
@@ -221,35 +217,27 @@ struct PairLDS {
 //       wave reduction; a bead split between two waves gets the later wave's
 //       share through `head`, added in wave order, so results are
 //       deterministic.  Then the chain rule for the NL force of each bead.
-template <int MM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
-    DProt rec, DProt lig, DGrid rInner, DGrid rOuter, DNL nl, DPar par, DBasis bin, DBasis bout,
-    const float* __restrict__ dofs, unsigned dofStride, const float* __restrict__ wts,
-    float* __restrict__ out, unsigned outStride, float* __restrict__ pgPartial, unsigned n,
-    unsigned tabSlots) {
-    extern __shared__ att_pair_params s_dyn[];
-    __shared__ float s_red[kAdjWaves][13 + 2 * ATT_MODES_MAX];
-    __shared__ PairLDS S;
-    const att_pair_params* tab = stage_table(par, s_dyn);
-    uint2* s_nl = reinterpret_cast<uint2*>(reinterpret_cast<char*>(s_dyn) +
-                                           adjoint_nl_offset(tabSlots));  // per bead {count, start}
-    unsigned short* s_queue = reinterpret_cast<unsigned short*>(s_nl + lig.n);
-    float* s_pg = pgPartial ? reinterpret_cast<float*>(s_queue + ((lig.n + 1) & ~1u)) : nullptr;
-    const unsigned npg = 3 * par.T * par.T;
-    if (s_pg)
-        for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
+template <int MM, int WAVES>
+__device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, PairLDS<WAVES>& S,
+                                             char* dyn, const att_pair_params* tab, float* s_pg) {
+    const DProt& rec = P.rec;
+    const DProt& lig = P.lig;
+    const DNL& nl = P.nl;
+    const DPar& par = P.par;
+    uint2* s_nl = reinterpret_cast<uint2*>(dyn + adjoint_nl_offset(P.tabSlots));
+    unsigned short* s_queue = reinterpret_cast<unsigned short*>(dyn + adjoint_queue_offset(P.tabSlots, lig.n));
+    constexpr unsigned kSlots = PairLDS<WAVES>::kSlots;
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned wave = threadIdx.x / kWave;
     constexpr int MA = MM > 0 ? MM : 1;
     const float4* __restrict__ rpos = rec.pos;
     const float4* __restrict__ rmodesT = rec.modesT;
     const unsigned Mr = MM > 0 ? rec.M : 0, Ml = MM > 0 ? lig.M : 0, rowT = rec.rowT;
-
-    for (unsigned pose = blockIdx.x; pose < n; pose += gridDim.x) {
+    {
         __syncthreads();  // LDS of the previous pose is no longer read
-        const float* d = dofs + (size_t)dofStride * pose;
+        const float* d = P.dofs + (size_t)P.dofStride * pose;
         const float tx = d[0], ty = d[1], tz = d[2];
-        const float wp = wts ? wts[pose] : 1.f;
+        const float wp = P.wts ? P.wts[pose] : 1.f;
         const PGrad pg{s_pg, tab, par.T, wp};
         float ampR[MA], ampL[MA];
 #pragma unroll
@@ -302,7 +290,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             V3 xt;
             const V3 p = place(i, x, xt);
             float3 G;
-            acc[0] += grid_adj(rInner, rOuter, bin, bout, par, p.x, p.y, p.z, lig.mtype[i], x.w, G, pg);
+            acc[0] += grid_adj(P.inner, P.outer, P.bin, P.bout, par, p.x, p.y, p.z, lig.mtype[i], x.w, G, pg);
             add_grad(i, xt.x, xt.y, xt.z, G.x, G.y, G.z);
             s_nl[i] = nl_cell(nl, p.x, p.y, p.z, lig.type[i]);
         }
@@ -339,15 +327,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 S.start[threadIdx.x] = c.y;
                 S.G[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            unsigned P;
-            const unsigned ex = block_scan(cnt, S.scan, P);
+            unsigned NP;
+            const unsigned ex = block_scan(cnt, S.scan, NP);
             if (threadIdx.x < ns) S.end[threadIdx.x] = ex + cnt;
             if (lane == 0) S.head[wave] = make_float4(0.f, 0.f, 0.f, 0.f);
             __syncthreads();
 
             // this wave's range of pairs and the slot it starts in
-            const unsigned per = (P + kAdjWaves - 1) / kAdjWaves;
-            const unsigned rs = min(P, wave * per), re = min(P, rs + per);
+            const unsigned per = (NP + WAVES - 1) / WAVES;
+            const unsigned rs = min(NP, wave * per), re = min(NP, rs + per);
             unsigned b0 = 0;
             {
                 unsigned lo = 0, hi = ns;  // first slot whose end exceeds rs
@@ -472,9 +460,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             __syncthreads();
             if (threadIdx.x == 0) {
                 // shares of beads split between waves, in wave order
-                for (unsigned w = 1; w < (unsigned)kAdjWaves; ++w) {
-                    const unsigned ws = min(P, w * per);
-                    if (ws >= P) break;
+                for (unsigned w = 1; w < (unsigned)WAVES; ++w) {
+                    const unsigned ws = min(NP, w * per);
+                    if (ws >= NP) break;
                     unsigned lo = 0, hi = ns;
                     while (lo < hi) {
                         const unsigned mid = (lo + hi) >> 1;
@@ -504,11 +492,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         for (int m = 0; m < MA; ++m) { gR[m] = wave_sum(gR[m]); gL[m] = wave_sum(gL[m]); }
         if (lane == 0) {
 #pragma unroll
-            for (int k = 0; k < 13; ++k) s_red[wave][k] = acc[k];
+            for (int k = 0; k < 13; ++k) S.red[wave][k] = acc[k];
 #pragma unroll
             for (int m = 0; m < ATT_MODES_MAX; ++m) {
-                s_red[wave][13 + m] = m < MA ? gR[m < MA ? m : 0] : 0.f;
-                s_red[wave][13 + ATT_MODES_MAX + m] = m < MA ? gL[m < MA ? m : 0] : 0.f;
+                S.red[wave][13 + m] = m < MA ? gR[m < MA ? m : 0] : 0.f;
+                S.red[wave][13 + ATT_MODES_MAX + m] = m < MA ? gL[m < MA ? m : 0] : 0.f;
             }
         }
         __syncthreads();
@@ -516,17 +504,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             float v[13 + 2 * ATT_MODES_MAX];
 #pragma unroll
             for (int k = 0; k < 13 + 2 * ATT_MODES_MAX; ++k) {
-                float s = s_red[0][k];
-                for (int ww = 1; ww < kAdjWaves; ++ww) s += s_red[ww][k];
+                float s = S.red[0][k];
+                for (int ww = 1; ww < WAVES; ++ww) s += S.red[ww][k];
                 v[k] = s;
             }
             float da[3];
             torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], v + 4, da);
-            float* o = out + (size_t)outStride * pose;
+            float* o = P.out + (size_t)P.outStride * pose;
             o[0] = v[0];
             o[1] = wp * v[1]; o[2] = wp * v[2]; o[3] = wp * v[3];
             o[4] = wp * da[0]; o[5] = wp * da[1]; o[6] = wp * da[2];
-            if (outStride >= 27)
+            if (P.outStride >= 27)
 #pragma unroll
                 for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
                     o[7 + m] = m < Mr ? wp * v[13 + m] : 0.f;
@@ -534,11 +522,70 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 }
         }
     }
+}
+
+template <int MM, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
+    PairDesc P, float* __restrict__ pgPartial) {
+    extern __shared__ att_pair_params s_dyn[];
+    __shared__ PairLDS<WAVES> S;
+    char* dyn = reinterpret_cast<char*>(s_dyn);
+    const att_pair_params* tab = stage_table(P.par, s_dyn);
+    float* s_pg = pgPartial ? reinterpret_cast<float*>(dyn + adjoint_pg_offset(P.tabSlots, P.lig.n)) : nullptr;
+    const unsigned npg = 3 * P.par.T * P.par.T;
+    if (s_pg)
+        for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
+    for (unsigned pose = blockIdx.x; pose < P.n; pose += gridDim.x)
+        adjoint_pose<MM, WAVES>(P, pose, S, dyn, tab, s_pg);
     if (s_pg) {
         __syncthreads();
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x)
             pgPartial[(size_t)blockIdx.x * npg + k] = s_pg[k];
     }
+}
+
+// Batched adjoint over many pairs: persistent workgroups take tiles (pair,
+// first pose, count) from a shared counter; the pair's descriptor is read
+// with scalar loads and its table staged in LDS when the pair changes.
+template <int MM, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint_batch(
+    const PairDesc* __restrict__ pairs, const BatchTile* __restrict__ tiles, unsigned ntiles,
+    unsigned* __restrict__ counter) {
+    extern __shared__ att_pair_params s_dyn[];
+    __shared__ PairLDS<WAVES> S;
+    __shared__ unsigned s_tile;
+    char* dyn = reinterpret_cast<char*>(s_dyn);
+    unsigned cur = 0xFFFFFFFFu;
+    const att_pair_params* tab = nullptr;
+    for (;;) {
+        __syncthreads();  // s_tile and the table of the previous tile are no longer read
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const unsigned t = __builtin_amdgcn_readfirstlane(s_tile);
+        if (t >= ntiles) break;
+        const BatchTile tile = tiles[t];
+        const PairDesc& P = pairs[tile.pair];
+        if (tile.pair != cur) {
+            tab = stage_table(P.par, s_dyn);  // made visible by adjoint_pose's first barrier
+            cur = tile.pair;
+        }
+        for (unsigned pose = tile.first; pose < tile.first + tile.count; ++pose)
+            adjoint_pose<MM, WAVES>(P, pose, S, dyn, tab, nullptr);
+    }
+}
+
+__global__ void k_reset_counter(unsigned* counter) {
+    if (threadIdx.x == 0) counter[0] = 0u;
+}
+
+__global__ void k_fill_zero(float* p, unsigned n) {
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_zero, dim3(std::min(1024u, (n + 255) / 256)), dim3(256), 0, s, p, n);
+    return hipGetLastError();
 }
 
 // Sums the per-workgroup parameter-gradient rows in block order.
@@ -587,43 +634,58 @@ size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bo
     (void)rec;
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
-    return adjoint_nl_offset(slots) + sizeof(uint2) * (size_t)lig.n +
-           sizeof(unsigned short) * (((size_t)lig.n + 1) & ~(size_t)1) +
-           (pgrad ? sizeof(float) * 3 * (size_t)par.T * par.T : 0);
+    return adjoint_pg_offset(slots, lig.n) + (pgrad ? sizeof(float) * 3 * (size_t)par.T * par.T : 0);
 }
 
-unsigned adjoint_blocks_per_cu(unsigned mm, size_t lds) {
+size_t adjoint_static_lds(int waves) {
+    switch (waves) {
+        case 1: return sizeof(PairLDS<1>) + 16;
+        case 2: return sizeof(PairLDS<2>) + 16;
+        default: return sizeof(PairLDS<4>) + 16;
+    }
+}
+
+namespace {
+template <int MM>
+const void* adjoint_kernel(int waves, bool batch) {
+    switch (waves) {
+        case 1: return batch ? (const void*)k_adjoint_batch<MM, 1> : (const void*)k_adjoint<MM, 1>;
+        case 2: return batch ? (const void*)k_adjoint_batch<MM, 2> : (const void*)k_adjoint<MM, 2>;
+        default: return batch ? (const void*)k_adjoint_batch<MM, 4> : (const void*)k_adjoint<MM, 4>;
+    }
+}
+const void* adjoint_kernel_for(unsigned mm, int waves, bool batch) {
+    if (mm == 0) return adjoint_kernel<0>(waves, batch);
+    if (mm <= 5) return adjoint_kernel<5>(waves, batch);
+    return adjoint_kernel<ATT_MODES_MAX>(waves, batch);
+}
+}  // namespace
+
+unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds) {
     int b = 0;
-    hipError_t e;
-    if (mm == 0) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adjoint<0>, 256, lds);
-    else if (mm <= 5) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adjoint<5>, 256, lds);
-    else e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_adjoint<ATT_MODES_MAX>, 256, lds);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &b, adjoint_kernel_for(mm, waves, batch), 64 * waves, lds);
     return e == hipSuccess && b > 0 ? (unsigned)b : 1u;
 }
 
-size_t adjoint_static_lds() { return sizeof(PairLDS) + sizeof(float) * kAdjWaves * (13 + 2 * ATT_MODES_MAX); }
+hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsigned blocks,
+                          int waves, size_t lds, hipStream_t s) {
+    if (P.n == 0) return hipSuccess;
+    const unsigned mm = modes ? (P.rec.M > P.lig.M ? P.rec.M : P.lig.M) : 0;
+    void* args[] = {const_cast<PairDesc*>(&P), &pgPartial};
+    return hipLaunchKernel(adjoint_kernel_for(mm, waves, false), dim3(blocks), dim3(64 * waves),
+                           args, lds, s);
+}
 
-hipError_t launch_adjoint(const DProt& rec, const DProt& lig, const DGrid& inner,
-                          const DGrid& outer, const DNL& nl, const DPar& par, const DBasis& bin,
-                          const DBasis& bout, const float* dofs, bool modes, const float* w,
-                          float* out, float* pgPartial, unsigned blocks, unsigned n,
-                          hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    unsigned slots = 0;
-    const size_t lds = adjoint_lds_bytes(rec, lig, par, pgPartial != nullptr, &slots);
-    const unsigned ds = modes ? 26 : 6, os = modes ? 27 : 7;
-    const unsigned mm = modes ? (rec.M > lig.M ? rec.M : lig.M) : 0;
-    if (mm == 0)
-        hipLaunchKernelGGL(k_adjoint<0>, dim3(blocks), dim3(256), lds, s, rec, lig, inner, outer,
-                           nl, par, bin, bout, dofs, ds, w, out, os, pgPartial, n, slots);
-    else if (mm <= 5)
-        hipLaunchKernelGGL(k_adjoint<5>, dim3(blocks), dim3(256), lds, s, rec, lig, inner, outer,
-                           nl, par, bin, bout, dofs, ds, w, out, os, pgPartial, n, slots);
-    else
-        hipLaunchKernelGGL(k_adjoint<ATT_MODES_MAX>, dim3(blocks), dim3(256), lds, s, rec, lig,
-                           inner, outer, nl, par, bin, bout, dofs, ds, w, out, os, pgPartial, n,
-                           slots);
-    return hipGetLastError();
+hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,
+                                unsigned ntiles, unsigned* counter, unsigned blocks, size_t lds,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || ntiles == 0) return e;
+    void* args[] = {&pairs, &tiles, &ntiles, &counter};
+    return hipLaunchKernel(adjoint_kernel_for(mm, waves, true), dim3(blocks), dim3(64 * waves), args,
+                           lds, s);
 }
 
 hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, float* pg,

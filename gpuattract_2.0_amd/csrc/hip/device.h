@@ -458,6 +458,38 @@ __device__ __forceinline__ const att_pair_params* stage_table(const DPar& par,
     return par.tab;
 }
 
+// Byte offset of the per-bead neighbour-list cells behind the pair table.
+__host__ __device__ constexpr size_t adjoint_nl_offset(unsigned tabSlots) {
+    return (sizeof(att_pair_params) * (size_t)tabSlots + 15) & ~(size_t)15;
+}
+
+// Everything the adjoint of one pose needs about its receptor-ligand pair.
+struct PairDesc {
+    DProt rec, lig;
+    DGrid inner, outer;
+    DNL nl;
+    DPar par;
+    DBasis bin, bout;
+    const float* dofs;   // n * dofStride floats
+    const float* wts;    // n upstream weights, or nullptr (all 1)
+    float* out;          // n * outStride floats
+    unsigned dofStride, outStride, n, tabSlots;
+};
+
+// Dynamic LDS of a pair: pair table | per-bead {count, start} | queue of
+// bead indices (u16) | parameter-gradient accumulators.
+__host__ __device__ inline size_t adjoint_queue_offset(unsigned tabSlots, unsigned ligN) {
+    return adjoint_nl_offset(tabSlots) + 8 * (size_t)ligN;
+}
+__host__ __device__ inline size_t adjoint_pg_offset(unsigned tabSlots, unsigned ligN) {
+    return adjoint_queue_offset(tabSlots, ligN) + 2 * (((size_t)ligN + 1) & ~(size_t)1);
+}
+
+// One unit of batched work: `count` consecutive poses of pair `pair`.
+struct BatchTile {
+    unsigned pair, first, count;
+};
+
 struct NoReaction {
     __device__ void operator()(unsigned, float, float, float) const {}
 };

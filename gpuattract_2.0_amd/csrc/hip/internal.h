@@ -38,16 +38,22 @@ hipError_t launch_octets(const float4* grid, float4* oct, unsigned nx, unsigned 
 
 size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool pgrad,
                          unsigned* tabSlots);
-// Static LDS of k_adjoint (pair-phase slots, reductions), and how many of its
-// 256-thread workgroups one CU holds with `lds` bytes of dynamic LDS.
-size_t adjoint_static_lds();
-unsigned adjoint_blocks_per_cu(unsigned mm, size_t lds);
+// Static LDS of the adjoint kernels with `waves` waves per pose, and how many
+// of their workgroups one CU holds with `lds` bytes of dynamic LDS.
+size_t adjoint_static_lds(int waves);
+unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds);
 
-hipError_t launch_adjoint(const DProt& rec, const DProt& lig, const DGrid& inner,
-                          const DGrid& outer, const DNL& nl, const DPar& par, const DBasis& bin,
-                          const DBasis& bout, const float* dofs, bool modes, const float* w,
-                          float* out, float* pgPartial, unsigned blocks, unsigned n,
-                          hipStream_t s);
+// One pair: persistent grid of `blocks` workgroups of 64*waves threads.
+hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsigned blocks,
+                          int waves, size_t lds, hipStream_t s);
+// Many pairs of one mode-capacity class: resets the tile counter, then the
+// persistent batch kernel (graph-capturable: no memset, no host sync).
+hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,
+                                unsigned ntiles, unsigned* counter, unsigned blocks, size_t lds,
+                                hipStream_t s);
+
+// Zero-fills n floats with a kernel (graph-capturable, unlike hipMemsetAsync).
+hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s);
 
 hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, float* pg,
                                hipStream_t s);

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <unordered_map>
@@ -588,24 +589,33 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     const bool pgrad = d_pgrad != nullptr;
     if (pgrad && (!g->bin.oct || g->bin.nb != 2 * p->d.T))
         return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
-    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, nullptr);
-    if (dyn + adjoint_static_lds() > c->maxLdsPerBlock)
+    unsigned slots = 0;
+    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, &slots);
+    constexpr int kWaves = 4;  // one 256-thread workgroup per pose
+    if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const unsigned cols = 3 * p->d.T * p->d.T;
     // persistent grid: as many workgroups as the device holds at once
     const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
-    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, dyn);
+    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
     if (pgrad && (st = grow(c, &c->d_pg, &c->d_pg_cap, sizeof(float) * (size_t)blocks * cols)) != ATT_OK)
         return st;
     if (n == 0) {
-        if (pgrad) ATT_HIP(c, hipMemsetAsync(d_pgrad, 0, sizeof(float) * cols, s));
+        if (pgrad) ATT_HIP(c, launch_fill_zero(d_pgrad, cols, s));
         return ATT_OK;
     }
-    ATT_HIP(c, launch_adjoint(rec->d, lig->d, g->inner, g->outer, g->nl, p->d, g->bin, g->bout,
-                              d_dofs, modes != 0, d_w, d_out,
-                              pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks, (unsigned)n, s));
+    PairDesc P{};
+    P.rec = rec->d; P.lig = lig->d;
+    P.inner = g->inner; P.outer = g->outer; P.nl = g->nl;
+    P.par = p->d;
+    P.bin = g->bin; P.bout = g->bout;
+    P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
+    P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
+    P.n = (unsigned)n; P.tabSlots = slots;
+    ATT_HIP(c, launch_adjoint(P, modes != 0, pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks,
+                              kWaves, dyn, s));
     if (pgrad)
         ATT_HIP(c, launch_pgrad_reduce(static_cast<const float*>(c->d_pg), blocks, cols, d_pgrad, s));
     return ATT_OK;
@@ -871,6 +881,261 @@ att_status att_event_query(att_ctx* c, void* ev) {
 att_status att_event_sync(att_ctx* c, void* ev) {
     if (!c || !ev) return fail(c, ATT_ERR_INVALID, "null argument");
     ATT_HIP(c, hipEventSynchronize((hipEvent_t)ev));
+    return ATT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Many receptor-ligand pairs of uneven sizes scored by a few persistent
+// launches.  Pairs are grouped into buckets by mode capacity (0, <=5, <=10)
+// and ligand size (<192, <640, larger beads); each bucket runs one
+// k_adjoint_batch launch whose shape (waves per pose, poses per tile) the
+// autotuner picks by timing the candidates on the bucket's own work.  The
+// launches are captured into one hipGraph, replayed by att_batch_run.
+struct att_batch {
+    struct Bucket {
+        unsigned mm = 0;
+        int sizeClass = 0;
+        std::vector<unsigned> pairs;  // indices into att_batch::pairs
+        size_t lds = 0;               // dynamic LDS: the largest pair of the bucket
+        int waves = 4;
+        unsigned tile = 16;
+        std::vector<BatchTile> tiles;
+        BatchTile* d_tiles = nullptr;
+        size_t d_tiles_cap = 0;
+        float ms = -1.f;              // time of the chosen shape when tuned
+    };
+    std::vector<PairDesc> pairs;
+    std::vector<double> cost;         // per pair, relative cost of one pose
+    PairDesc* d_pairs = nullptr;
+    std::vector<Bucket> buckets;
+    unsigned* d_counters = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+};
+
+namespace {
+
+int size_class(unsigned ligN) { return ligN < 192 ? 0 : ligN < 640 ? 1 : 2; }
+
+att_status batch_tiles(att_ctx* c, att_batch* b, att_batch::Bucket& k) {
+    k.tiles.clear();
+    std::vector<unsigned> order = k.pairs;
+    // expensive pairs first, so the last tiles taken are the cheap ones
+    std::stable_sort(order.begin(), order.end(),
+                     [&](unsigned x, unsigned y) { return b->cost[x] > b->cost[y]; });
+    for (unsigned p : order) {
+        const unsigned n = b->pairs[p].n;
+        for (unsigned f = 0; f < n; f += k.tile)
+            k.tiles.push_back(BatchTile{p, f, std::min(k.tile, n - f)});
+    }
+    const size_t bytes = sizeof(BatchTile) * std::max<size_t>(k.tiles.size(), 1);
+    att_status st;
+    if ((st = grow(c, reinterpret_cast<void**>(&k.d_tiles), &k.d_tiles_cap, bytes)) != ATT_OK) return st;
+    ATT_HIP(c, hipMemcpy(k.d_tiles, k.tiles.data(), sizeof(BatchTile) * k.tiles.size(),
+                         hipMemcpyHostToDevice));
+    b->dirty = true;
+    return ATT_OK;
+}
+
+hipError_t batch_launch(att_ctx* c, att_batch* b, size_t i, hipStream_t s) {
+    att_batch::Bucket& k = b->buckets[i];
+    const size_t resident =
+        (size_t)c->numCU * adjoint_blocks_per_cu(k.mm, k.waves, true, k.lds);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(k.tiles.size(), resident));
+    return launch_adjoint_batch(k.mm, k.waves, b->d_pairs, k.d_tiles, (unsigned)k.tiles.size(),
+                                b->d_counters + i, blocks, k.lds, s);
+}
+
+void batch_free_graph(att_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+att_status att_batch_create(att_ctx* c, const att_batch_pair* in, unsigned npairs, att_batch** out) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!out || (npairs && !in)) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    std::unique_ptr<att_batch> b(new att_batch);
+    std::map<std::pair<unsigned, int>, size_t> index;
+    for (unsigned i = 0; i < npairs; ++i) {
+        const att_batch_pair& q = in[i];
+        ProtObj *rec, *lig;
+        GridObj* g;
+        ParObj* p;
+        att_status st;
+        if ((st = resolve(c, q.rec, q.lig, q.grid, q.params, &rec, &lig, &g, &p)) != ATT_OK) return st;
+        if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
+        if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
+        if (q.n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one pair");
+        if (q.n && (!q.d_dofs || !q.d_out)) return fail(c, ATT_ERR_INVALID, "null buffer");
+        unsigned slots = 0;
+        const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
+        if (dyn + adjoint_static_lds(4) > c->maxLdsPerBlock)
+            return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
+        PairDesc P{};
+        P.rec = rec->d; P.lig = lig->d;
+        P.inner = g->inner; P.outer = g->outer; P.nl = g->nl;
+        P.par = p->d;
+        P.dofs = q.d_dofs; P.wts = q.d_w; P.out = q.d_out;
+        P.dofStride = q.modes ? 26 : 6; P.outStride = q.modes ? 27 : 7;
+        P.n = (unsigned)q.n; P.tabSlots = slots;
+        const unsigned m = q.modes ? std::max(rec->d.M, lig->d.M) : 0;
+        const unsigned mm = m == 0 ? 0 : m <= 5 ? 5 : ATT_MODES_MAX;
+        const int sc = size_class(lig->d.n);
+        auto key = std::make_pair(mm, sc);
+        auto it = index.find(key);
+        if (it == index.end()) {
+            it = index.emplace(key, b->buckets.size()).first;
+            b->buckets.emplace_back();
+            b->buckets.back().mm = mm;
+            b->buckets.back().sizeClass = sc;
+        }
+        att_batch::Bucket& k = b->buckets[it->second];
+        k.pairs.push_back(i);
+        k.lds = std::max(k.lds, dyn);
+        b->pairs.push_back(P);
+        b->cost.push_back((double)lig->d.n * (1.0 + 0.25 * m) + 0.05 * rec->d.n);
+    }
+    att_status st;
+    if (npairs) {
+        ATT_HIP(c, hipMalloc(&b->d_pairs, sizeof(PairDesc) * npairs));
+        ATT_HIP(c, hipMemcpy(b->d_pairs, b->pairs.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice));
+    }
+    ATT_HIP(c, hipMalloc(&b->d_counters, sizeof(unsigned) * std::max<size_t>(b->buckets.size(), 1)));
+    for (auto& k : b->buckets)
+        if ((st = batch_tiles(c, b.get(), k)) != ATT_OK) return st;
+    ATT_HIP(c, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
+    *out = b.release();
+    return ATT_OK;
+}
+
+att_status att_batch_destroy(att_ctx* c, att_batch* b) {
+    if (!c || !b) return ATT_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    batch_free_graph(b);
+    for (auto& k : b->buckets)
+        if (k.d_tiles) (void)hipFree(k.d_tiles);
+    if (b->d_pairs) (void)hipFree(b->d_pairs);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
+    delete b;
+    return ATT_OK;
+}
+
+unsigned att_batch_buckets(const att_batch* b) { return b ? (unsigned)b->buckets.size() : 0u; }
+
+att_status att_batch_bucket_info(att_ctx* c, const att_batch* b, unsigned bucket, att_batch_bucket* info) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!b || !info || bucket >= b->buckets.size()) return fail(c, ATT_ERR_INVALID, "bad bucket");
+    const att_batch::Bucket& k = b->buckets[bucket];
+    info->modeCapacity = k.mm;
+    info->sizeClass = k.sizeClass;
+    info->numPairs = (unsigned)k.pairs.size();
+    info->numTiles = (unsigned)k.tiles.size();
+    info->wavesPerPose = k.waves;
+    info->posesPerTile = k.tile;
+    info->ms = k.ms;
+    return ATT_OK;
+}
+
+att_status att_batch_set_shape(att_ctx* c, att_batch* b, unsigned bucket, int wavesPerPose,
+                               unsigned posesPerTile) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!b || bucket >= b->buckets.size()) return fail(c, ATT_ERR_INVALID, "bad bucket");
+    if ((wavesPerPose != 1 && wavesPerPose != 2 && wavesPerPose != 4) || posesPerTile == 0)
+        return fail(c, ATT_ERR_INVALID, "waves per pose must be 1, 2 or 4 and tiles non-empty");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_batch::Bucket& k = b->buckets[bucket];
+    if (k.lds + adjoint_static_lds(wavesPerPose) > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "shape does not fit the LDS");
+    k.waves = wavesPerPose;
+    k.ms = -1.f;
+    if (k.tile != posesPerTile || k.tiles.empty()) {
+        k.tile = posesPerTile;
+        att_status st;
+        if ((st = batch_tiles(c, b, k)) != ATT_OK) return st;
+    }
+    b->dirty = true;
+    return ATT_OK;
+}
+
+att_status att_batch_tune(att_ctx* c, att_batch* b, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!b) return fail(c, ATT_ERR_INVALID, "null batch");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    hipEvent_t e0, e1;
+    ATT_HIP(c, hipEventCreate(&e0));
+    ATT_HIP(c, hipEventCreate(&e1));
+    static const int kWaves[] = {1, 2, 4};
+    static const unsigned kTiles[] = {4, 16, 64};
+    att_status st = ATT_OK;
+    for (size_t i = 0; i < b->buckets.size() && st == ATT_OK; ++i) {
+        att_batch::Bucket& k = b->buckets[i];
+        float best = -1.f;
+        int bw = 4;
+        unsigned bt = 16;
+        for (int w : kWaves) {
+            if (k.lds + adjoint_static_lds(w) > c->maxLdsPerBlock) continue;
+            for (unsigned tl : kTiles) {
+                if ((st = att_batch_set_shape(c, b, (unsigned)i, w, tl)) != ATT_OK) break;
+                hipError_t e = batch_launch(c, b, i, s);  // warm-up
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventRecord(e0, s);
+                if (e == hipSuccess) e = batch_launch(c, b, i, s);
+                if (e == hipSuccess) e = batch_launch(c, b, i, s);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+                if (e != hipSuccess) {
+                    st = fail(c, ATT_ERR_HIP, std::string("autotuner: ") + hipGetErrorString(e));
+                    break;
+                }
+                ms *= 0.5f;
+                if (best < 0.f || ms < best) { best = ms; bw = w; bt = tl; }
+            }
+            if (st != ATT_OK) break;
+        }
+        if (st == ATT_OK && (st = att_batch_set_shape(c, b, (unsigned)i, bw, bt)) == ATT_OK) k.ms = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+}
+
+att_status att_batch_run(att_ctx* c, att_batch* b, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!b) return fail(c, ATT_ERR_INVALID, "null batch");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    if (b->buckets.empty()) return ATT_OK;
+    if (b->dirty) {
+        batch_free_graph(b);
+        ATT_HIP(c, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+        hipError_t e = hipSuccess;
+        for (size_t i = 0; i < b->buckets.size() && e == hipSuccess; ++i) e = batch_launch(c, b, i, b->capture);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(b->capture, &g);
+        if (e != hipSuccess || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(c, ATT_ERR_HIP, std::string("batch capture: ") +
+                                            hipGetErrorString(e != hipSuccess ? e : ee));
+        }
+        b->graph = g;
+        ATT_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->dirty = false;
+    }
+    ATT_HIP(c, hipGraphLaunch(b->exec, s));
     return ATT_OK;
 }
 

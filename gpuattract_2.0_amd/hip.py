@@ -39,6 +39,18 @@ class NLDesc(C.Structure):
                 ("plateau", C.c_float), ("desc", P), ("list", P), ("numEl", C.c_size_t)]
 
 
+class BatchPair(C.Structure):
+    _fields_ = [("rec", C.c_uint64), ("lig", C.c_uint64), ("grid", C.c_uint64),
+                ("params", C.c_uint64), ("d_dofs", P), ("d_out", P), ("d_w", P),
+                ("n", C.c_size_t), ("modes", C.c_int)]
+
+
+class BatchBucket(C.Structure):
+    _fields_ = [("modeCapacity", C.c_uint), ("sizeClass", C.c_int), ("numPairs", C.c_uint),
+                ("numTiles", C.c_uint), ("wavesPerPose", C.c_int), ("posesPerTile", C.c_uint),
+                ("ms", C.c_float)]
+
+
 PAIR_DTYPE = np.dtype([("rc", np.float32), ("ac", np.float32), ("ipon", np.int32),
                        ("rmin2", np.float32), ("emin", np.float32)])
 
@@ -119,6 +131,20 @@ def lib():
         L.att_upload_basis.argtypes = [P, H, C.c_uint, P, P]
         L.att_score_adjoint.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, P, P, P, P]
         L.att_score_adjoint_dev.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, P, P, P, P]
+        L.att_stream_create.argtypes = [P, C.POINTER(P)]
+        L.att_stream_destroy.argtypes = [P, P]
+        L.att_memcpy_h2d_async.argtypes = [P, P, P, C.c_size_t, P]
+        L.att_memcpy_d2h_async.argtypes = [P, P, P, C.c_size_t, P]
+        L.att_event_query.argtypes = [P, P]
+        L.att_event_sync.argtypes = [P, P]
+        L.att_batch_create.argtypes = [P, C.POINTER(BatchPair), C.c_uint, C.POINTER(P)]
+        L.att_batch_destroy.argtypes = [P, P]
+        L.att_batch_buckets.argtypes = [P]
+        L.att_batch_buckets.restype = C.c_uint
+        L.att_batch_bucket_info.argtypes = [P, P, C.c_uint, C.POINTER(BatchBucket)]
+        L.att_batch_set_shape.argtypes = [P, P, C.c_uint, C.c_int, C.c_uint]
+        L.att_batch_tune.argtypes = [P, P, P]
+        L.att_batch_run.argtypes = [P, P, P]
         _lib = L
     return _lib
 
@@ -143,6 +169,43 @@ class DeviceBuffer:
             self.release()
         except Exception:
             pass
+
+
+class Batch:
+    """att_batch: many pairs scored by graph-captured persistent launches."""
+
+    def __init__(self, ctx, pairs):
+        self.ctx = ctx
+        arr = (BatchPair * len(pairs))()
+        for k, q in enumerate(pairs):
+            hs = q["hs"]
+            arr[k] = BatchPair(hs["rec"], hs["lig"], hs["grid"], hs["params"], q["d_dofs"],
+                               q["d_out"], q.get("d_w"), q["n"], 1 if q["modes"] else 0)
+        h = P()
+        ctx._chk(ctx.L.att_batch_create(ctx.h, arr, len(pairs), C.byref(h)))
+        self.h = h.value
+
+    def buckets(self):
+        out = []
+        for b in range(self.ctx.L.att_batch_buckets(self.h)):
+            info = BatchBucket()
+            self.ctx._chk(self.ctx.L.att_batch_bucket_info(self.ctx.h, self.h, b, C.byref(info)))
+            out.append({f: getattr(info, f) for f, _ in BatchBucket._fields_})
+        return out
+
+    def set_shape(self, bucket, waves, tile):
+        self.ctx._chk(self.ctx.L.att_batch_set_shape(self.ctx.h, self.h, bucket, waves, tile))
+
+    def tune(self, stream=None):
+        self.ctx._chk(self.ctx.L.att_batch_tune(self.ctx.h, self.h, stream))
+
+    def run(self, stream=None):
+        self.ctx._chk(self.ctx.L.att_batch_run(self.ctx.h, self.h, stream))
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.ctx.L.att_batch_destroy(self.ctx.h, self.h)
+        self.h = None
 
 
 class Context:
@@ -255,6 +318,10 @@ class Context:
 
     def sync(self, stream=None):
         self._chk(self.L.att_sync(self.h, stream))
+
+    def batch(self, pairs):
+        """pairs: dicts {hs, d_dofs, d_out, n, modes[, d_w]} (device pointers)."""
+        return Batch(self, pairs)
 
     # -- device buffers and timing ----------------------------------------
     def dev_alloc(self, nbytes):

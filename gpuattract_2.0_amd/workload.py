@@ -328,6 +328,7 @@ def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_ty
         glo = cube_geom((0.0, 0.0, 0.0), r_hi + R_rec + 2.0, outer_spacing)
         wl.lig_outer = build_grid(glo, lig, params, True, plateau)
         wl.lig_nl = build_nl(gli, lig, neighbourdis, plateau)
+    if modes or M_rec > 0 or M_lig > 0:
         dof = np.zeros((n_poses, 26), np.float32)
         dof[:, :6] = dofs6
         dof[:, 6:6 + M_rec] = make_amplitudes(seed, n_poses, M_rec, entity=5)
@@ -349,6 +350,37 @@ def config_rigid_1000_300(n_poses=10000):
 
 def config_modes_3000_800(n_poses=100000):
     return make_workload("modes 3000/800 5+5", 20261203, 3000, 800, n_poses, 5, 5)
+
+
+def make_batch(seed, n_pairs, rec_range=(500, 5000), lig_range=(100, 1500), modes_range=(0, 10),
+               poses_range=(1000, 50000), progress=None, **kw):
+    """Pairs of uneven sizes for batched scoring: receptor and ligand bead
+    counts and pose counts log-uniform in their ranges, each body's number of
+    modes uniform in modes_range; pair k uses the master seed plus 1000 * (k+1)
+    for its own bodies, table, grids and poses.  Ligand grids are not built
+    (the adjoint does not use them); DOFs carry the modes layout when either
+    body has modes."""
+    def logu(entity, k, lo, hi):
+        u = gen_lib().attg_uniform(seed, entity, k)
+        return int(round(math.exp(math.log(lo) + u * (math.log(hi) - math.log(lo)))))
+
+    def uni(entity, k, lo, hi):
+        u = gen_lib().attg_uniform(seed, entity, k)
+        return min(hi, lo + int(u * (hi - lo + 1)))
+
+    pairs = []
+    for k in range(n_pairs):
+        n_rec = logu(60, 4 * k, *rec_range)
+        n_lig = logu(60, 4 * k + 1, *lig_range)
+        n_poses = logu(60, 4 * k + 2, *poses_range)
+        M_rec = uni(61, 2 * k, *modes_range)
+        M_lig = uni(61, 2 * k + 1, *modes_range)
+        pairs.append(make_workload(f"pair {k}: {n_rec}/{n_lig} beads, {M_rec}+{M_lig} modes, "
+                                   f"{n_poses} poses", seed + 1000 * (k + 1), n_rec, n_lig,
+                                   n_poses, M_rec, M_lig, with_lig_grid=False, **kw))
+        if progress:
+            progress(k + 1, pairs[-1])
+    return pairs
 
 
 def checksum(*arrays):

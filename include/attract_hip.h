@@ -215,6 +215,42 @@ att_status att_memcpy_d2h_async(att_ctx* ctx, void* dst, const void* d_src, size
 att_status att_event_query(att_ctx* ctx, void* ev);
 att_status att_event_sync(att_ctx* ctx, void* ev);
 
+/* --- batched adjoint over many pairs --------------------------------------
+ * No reference counterpart: the reference scores one receptor-ligand pair per
+ * request (GPUEnergyService6D::createItemProcessor, GPUEnergyService6D.tpp:
+ * 468-520, items of 100 poses).  A batch scores many pairs of uneven bead and
+ * pose counts with a few persistent launches: pairs are grouped into buckets
+ * by mode capacity and ligand size, each bucket's launch shape (waves per
+ * pose, poses per work tile) is picked by att_batch_tune, and the launches are
+ * captured into one hipGraph that att_batch_run replays.  Per pose the output
+ * is the adjoint's (att_score_adjoint_dev without parameter gradients).  The
+ * device buffers named by the pairs must stay valid while the batch exists. */
+typedef struct att_batch att_batch;
+typedef struct {
+    att_handle rec, lig, grid, params;  /* uploaded receptor, ligand, receptor grid, table */
+    const float* d_dofs;                /* n * (modes ? 26 : 6) floats */
+    float* d_out;                       /* n * (modes ? 27 : 7) floats */
+    const float* d_w;                   /* n upstream weights, or NULL */
+    size_t n;
+    int modes;
+} att_batch_pair;
+typedef struct {
+    unsigned modeCapacity;   /* 0, 5 or 10 */
+    int sizeClass;           /* ligand beads: 0 (<192), 1 (<640), 2 */
+    unsigned numPairs, numTiles;
+    int wavesPerPose;        /* 1, 2 or 4 */
+    unsigned posesPerTile;
+    float ms;                /* time of the bucket's launch when tuned, else -1 */
+} att_batch_bucket;
+att_status att_batch_create(att_ctx* ctx, const att_batch_pair* pairs, unsigned npairs, att_batch** out);
+att_status att_batch_destroy(att_ctx* ctx, att_batch* b);
+unsigned att_batch_buckets(const att_batch* b);
+att_status att_batch_bucket_info(att_ctx* ctx, const att_batch* b, unsigned bucket, att_batch_bucket* info);
+att_status att_batch_set_shape(att_ctx* ctx, att_batch* b, unsigned bucket, int wavesPerPose,
+                               unsigned posesPerTile);
+att_status att_batch_tune(att_ctx* ctx, att_batch* b, void* stream);
+att_status att_batch_run(att_ctx* ctx, att_batch* b, void* stream);
+
 /* Library build identification (arch, flags). */
 const char* att_build_info(void);
 
