@@ -315,7 +315,7 @@ def run_batched(args, np, hip, traffic, W):
         import oracle_py as O
         from concurrent.futures import ThreadPoolExecutor
         threads = max(1, min(16, os.cpu_count() or 1))
-        k = 24
+        k = 160
         t0 = time.perf_counter()
         with ThreadPoolExecutor(threads) as ex:
             list(ex.map(lambda s: O.adjoint(s["wl"], s["wl"].dofs[:k], f32=True), staged))

@@ -1078,7 +1078,7 @@ att_status att_batch_tune(att_ctx* c, att_batch* b, void* stream) {
     ATT_HIP(c, hipEventCreate(&e0));
     ATT_HIP(c, hipEventCreate(&e1));
     static const int kWaves[] = {1, 2, 4};
-    static const unsigned kTiles[] = {4, 16, 64};
+    static const unsigned kTiles[] = {1, 2, 4, 16};
     att_status st = ATT_OK;
     for (size_t i = 0; i < b->buckets.size() && st == ATT_OK; ++i) {
         att_batch::Bucket& k = b->buckets[i];
