@@ -85,6 +85,10 @@ def build_service():
                     os.path.join(bin_dir, "server_semantics")])
         _run(cxx + ["-shared", os.path.join(tdir, "service_driver.cpp"), "-o",
                     os.path.join(bin_dir, "libservice_driver.so")] + link)
+        # the device minimizer's state machine, compiled for the host
+        _run([HIPCC] + HIP_FLAGS + ["-shared", "-I", os.path.join(CSRC, "hip"),
+                                    os.path.join(tdir, "va13_host.cpp"), "-o",
+                                    os.path.join(bin_dir, "libva13_host.so")])
     return out
 
 

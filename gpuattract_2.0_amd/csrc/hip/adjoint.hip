@@ -535,7 +535,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     const unsigned npg = 3 * P.par.T * P.par.T;
     if (s_pg)
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
-    for (unsigned pose = blockIdx.x; pose < P.n; pose += gridDim.x)
+    const unsigned n = P.nDev ? min(P.nDev[0], P.n) : P.n;
+    for (unsigned pose = blockIdx.x; pose < n; pose += gridDim.x)
         adjoint_pose<MM, WAVES>(P, pose, S, dyn, tab, s_pg);
     if (s_pg) {
         __syncthreads();

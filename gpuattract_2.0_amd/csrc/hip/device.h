@@ -474,6 +474,7 @@ struct PairDesc {
     const float* wts;    // n upstream weights, or nullptr (all 1)
     float* out;          // n * outStride floats
     unsigned dofStride, outStride, n, tabSlots;
+    const unsigned* nDev;  // when set, the number of poses is read here at launch (<= n)
 };
 
 // Dynamic LDS of a pair: pair table | per-bead {count, start} | queue of

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.h"
+#include "va13.h"
 
 namespace att {
 
@@ -51,6 +52,27 @@ hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsig
 hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,
                                 unsigned ntiles, unsigned* counter, unsigned blocks, size_t lds,
                                 hipStream_t s);
+
+// One device-resident minimization run (minimize.hip): per-pose VA13 states,
+// the next round's request list, count and points, and the results.
+struct MinRun {
+    double* st;        // npose * va_elements(dim), element-major
+    VaState* sc;       // npose
+    unsigned npose;
+    int dim, Mr, Ml, ivmax;
+    bool modes;
+    unsigned ds, os;   // DOF and result record widths (26/27 or 6/7)
+    unsigned* list;    // next round: pose of each request slot
+    unsigned* count;   // next round: number of requests
+    float* req;        // next round: requested points as DOF records
+    float* dofsOut;    // npose * ds: last evaluated DOFs
+    float* eLast;      // npose: their energy
+    float* eBest;      // npose or nullptr: lowest energy seen
+    int* nfun;         // npose or nullptr: line-search evaluations
+};
+hipError_t launch_min_start(const MinRun& R, const float* dofs, hipStream_t s);
+hipError_t launch_min_step(const MinRun& R, const unsigned* listIn, const unsigned* countIn,
+                           const float* reqIn, const float* res, hipStream_t s);
 
 // Zero-fills n floats with a kernel (graph-capturable, unlike hipMemsetAsync).
 hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s);

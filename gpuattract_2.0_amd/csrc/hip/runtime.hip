@@ -80,6 +80,8 @@ struct att_ctx {
     size_t d_w_cap = 0;
     void* d_g = nullptr;
     size_t d_g_cap = 0;
+    void* d_min = nullptr;  // minimizer workspace
+    size_t d_min_cap = 0;
 };
 
 namespace {
@@ -1136,6 +1138,128 @@ att_status att_batch_run(att_ctx* c, att_batch* b, void* stream) {
         b->dirty = false;
     }
     ATT_HIP(c, hipGraphLaunch(b->exec, s));
+    return ATT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- minimizer
+extern "C" {
+
+att_status att_minimize_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                            const float* d_dofs, int modes, size_t n, int maxFunEval,
+                            float* d_dofs_out, float* d_energy, float* d_ebest, int* d_nfun,
+                            void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if ((n && (!d_dofs || !d_dofs_out || !d_energy)) || maxFunEval < 0)
+        return fail(c, ATT_ERR_INVALID, "null buffer or negative evaluation limit");
+    if (n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one call");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* g;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
+    if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
+    if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
+    unsigned slots = 0;
+    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
+    constexpr int kWaves = 4;
+    if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
+    if (n == 0) return ATT_OK;
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+
+    MinRun R{};
+    R.npose = (unsigned)n;
+    R.Mr = modes ? (int)rec->d.M : 0;
+    R.Ml = modes ? (int)lig->d.M : 0;
+    R.dim = 6 + R.Mr + R.Ml;
+    R.ivmax = maxFunEval;
+    R.modes = modes != 0;
+    R.ds = modes ? 26 : 6;
+    R.os = modes ? 27 : 7;
+    // workspace: states, scalars, two request lists / counts / point sets, results
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bSt = al(sizeof(double) * n * va_elements(R.dim)), bSc = al(sizeof(VaState) * n),
+                 bList = al(sizeof(unsigned) * n), bCnt = al(2 * sizeof(unsigned)),
+                 bReq = al(sizeof(float) * n * R.ds), bRes = al(sizeof(float) * n * R.os);
+    if ((st = grow(c, &c->d_min, &c->d_min_cap, bSt + bSc + 2 * bList + bCnt + 2 * bReq + bRes)) != ATT_OK)
+        return st;
+    char* base = static_cast<char*>(c->d_min);
+    R.st = reinterpret_cast<double*>(base);
+    R.sc = reinterpret_cast<VaState*>(base + bSt);
+    unsigned* list[2] = {reinterpret_cast<unsigned*>(base + bSt + bSc),
+                         reinterpret_cast<unsigned*>(base + bSt + bSc + bList)};
+    unsigned* cnt = reinterpret_cast<unsigned*>(base + bSt + bSc + 2 * bList);
+    unsigned* count[2] = {cnt, cnt + 1};
+    float* req[2] = {reinterpret_cast<float*>(base + bSt + bSc + 2 * bList + bCnt),
+                     reinterpret_cast<float*>(base + bSt + bSc + 2 * bList + bCnt + bReq)};
+    float* res = reinterpret_cast<float*>(base + bSt + bSc + 2 * bList + bCnt + 2 * bReq);
+    R.dofsOut = d_dofs_out;
+    R.eLast = d_energy;
+    R.eBest = d_ebest;
+    R.nfun = d_nfun;
+
+    PairDesc P{};
+    P.rec = rec->d; P.lig = lig->d;
+    P.inner = g->inner; P.outer = g->outer; P.nl = g->nl;
+    P.par = p->d;
+    P.bin = g->bin; P.bout = g->bout;
+    P.out = res;
+    P.dofStride = R.ds; P.outStride = R.os;
+    P.n = (unsigned)n; P.tabSlots = slots;
+    const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
+    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
+
+    R.list = list[0]; R.count = count[0]; R.req = req[0];
+    ATT_HIP(c, launch_min_start(R, d_dofs, s));
+    int cur = 0;
+    // every pose is evaluated at most maxFunEval + 1 times
+    for (int round = 0; round <= maxFunEval + 1; ++round) {
+        P.dofs = req[cur];
+        P.nDev = count[cur];
+        ATT_HIP(c, launch_adjoint(P, modes != 0, nullptr, blocks, kWaves, dyn, s));
+        const int nxt = cur ^ 1;
+        R.list = list[nxt]; R.count = count[nxt]; R.req = req[nxt];
+        ATT_HIP(c, launch_min_step(R, list[cur], count[cur], req[cur], res, s));
+        cur = nxt;
+        if ((round & 7) == 7) {
+            unsigned running = 0;
+            ATT_HIP(c, hipMemcpyAsync(&running, count[cur], sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            ATT_HIP(c, hipStreamSynchronize(s));
+            if (running == 0) break;
+        }
+    }
+    return ATT_OK;
+}
+
+att_status att_minimize(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                        const float* dofs, int modes, size_t n, int maxFunEval, float* dofs_out,
+                        float* energy, float* ebest, int* nfun, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (n && (!dofs || !dofs_out || !energy)) return fail(c, ATT_ERR_INVALID, "null buffer");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const size_t ds = modes ? 26 : 6;
+    Tmp tmp;
+    float *d_in, *d_out, *d_e, *d_eb = nullptr;
+    int* d_nf = nullptr;
+    ATT_HIP(c, tmp.alloc(&d_in, n * ds));
+    ATT_HIP(c, tmp.alloc(&d_out, n * ds));
+    ATT_HIP(c, tmp.alloc(&d_e, n));
+    if (ebest) ATT_HIP(c, tmp.alloc(&d_eb, n));
+    if (nfun) ATT_HIP(c, tmp.alloc(&d_nf, n));
+    ATT_HIP(c, hipMemcpyAsync(d_in, dofs, sizeof(float) * n * ds, hipMemcpyHostToDevice, s));
+    att_status st = att_minimize_dev(c, hr, hl, hg, hp, d_in, modes, n, maxFunEval, d_out, d_e, d_eb,
+                                     d_nf, s);
+    if (st != ATT_OK) return st;
+    ATT_HIP(c, hipMemcpyAsync(dofs_out, d_out, sizeof(float) * n * ds, hipMemcpyDeviceToHost, s));
+    ATT_HIP(c, hipMemcpyAsync(energy, d_e, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (ebest) ATT_HIP(c, hipMemcpyAsync(ebest, d_eb, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    if (nfun) ATT_HIP(c, hipMemcpyAsync(nfun, d_nf, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    ATT_HIP(c, hipStreamSynchronize(s));
     return ATT_OK;
 }
 

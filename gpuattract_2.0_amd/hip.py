@@ -137,6 +137,9 @@ def lib():
         L.att_memcpy_d2h_async.argtypes = [P, P, P, C.c_size_t, P]
         L.att_event_query.argtypes = [P, P]
         L.att_event_sync.argtypes = [P, P]
+        L.att_minimize.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, C.c_int, P, P, P, P, P]
+        L.att_minimize_dev.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, C.c_int, P, P, P, P,
+                                       P]
         L.att_batch_create.argtypes = [P, C.POINTER(BatchPair), C.c_uint, C.POINTER(P)]
         L.att_batch_destroy.argtypes = [P, P]
         L.att_batch_buckets.argtypes = [P]
@@ -318,6 +321,24 @@ class Context:
 
     def sync(self, stream=None):
         self._chk(self.L.att_sync(self.h, stream))
+
+    def minimize(self, hs, dofs, max_fun_eval=500):
+        """VA13 from every pose (host arrays): (dofs_out, e_last, e_best, nfun)."""
+        d = np.ascontiguousarray(dofs, np.float32)
+        n, modes = d.shape[0], d.shape[1] == 26
+        out = np.zeros_like(d)
+        el, eb = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        nf = np.zeros(n, np.int32)
+        self._chk(self.L.att_minimize(self.h, hs["rec"], hs["lig"], hs["grid"], hs["params"],
+                                      _p(d), 1 if modes else 0, n, max_fun_eval, _p(out), _p(el),
+                                      _p(eb), _p(nf), None))
+        return out, el, eb, nf
+
+    def minimize_dev(self, hs, d_dofs, modes, n, max_fun_eval, d_out, d_energy, d_ebest=None,
+                     d_nfun=None, stream=None):
+        self._chk(self.L.att_minimize_dev(self.h, hs["rec"], hs["lig"], hs["grid"], hs["params"],
+                                          d_dofs, 1 if modes else 0, n, max_fun_eval, d_out,
+                                          d_energy, d_ebest, d_nfun, stream))
 
     def batch(self, pairs):
         """pairs: dicts {hs, d_dofs, d_out, n, modes[, d_w]} (device pointers)."""

@@ -251,6 +251,26 @@ att_status att_batch_set_shape(att_ctx* ctx, att_batch* b, unsigned bucket, int 
 att_status att_batch_tune(att_ctx* ctx, att_batch* b, void* stream);
 att_status att_batch_run(att_ctx* ctx, att_batch* b, void* stream);
 
+/* --- device-resident minimization -----------------------------------------
+ * Replaces the reference's minimization loop (apps/em/RequestHandler.tpp:
+ * 101-234 with VA13Solver, apps/em/VA13Solver.cpp:36-69, minfor.f): every
+ * pose runs VA13 with at most maxFunEval line-search evaluations (the
+ * reference's default is 500, VA13Solver.h:66) on the variables (angles,
+ * position, receptor modes, ligand modes); each round scores all running
+ * poses with the adjoint (energy and exact gradient).  Results per pose, as
+ * the reference's solver reports them: the last evaluated DOFs (same layout as
+ * the input) and their energy; optionally the lowest energy seen (ebest) and
+ * the number of line-search evaluations (nfun).  The _dev form takes device
+ * buffers and launches on `stream`; its host loop reads the number of running
+ * poses every eight rounds. */
+att_status att_minimize_dev(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                            att_handle params, const float* d_dofs, int modes, size_t n,
+                            int maxFunEval, float* d_dofs_out, float* d_energy, float* d_ebest,
+                            int* d_nfun, void* stream);
+att_status att_minimize(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                        att_handle params, const float* dofs, int modes, size_t n, int maxFunEval,
+                        float* dofs_out, float* energy, float* ebest, int* nfun, void* stream);
+
 /* Library build identification (arch, flags). */
 const char* att_build_info(void);
 

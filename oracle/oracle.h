@@ -129,6 +129,25 @@ int orc_adjoint_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* i
                     const float* inner_basis, const float* outer_basis, const float* dof,
                     size_t n, const double* w, double* out, double* scale, double* pgrad);
 
+/* VA13 variable-metric minimizer (apps/em/minfor.f, mc11a.f), any dimension.
+ * fg(ctx, x, &f, g) evaluates the objective and its gradient.  Reports the
+ * last evaluated point and value (as the reference's solver does) and the
+ * lowest value seen; nfun = line-search evaluations used (at most ivmax). */
+typedef void (*orc_fg)(void* ctx, const double* x, double* f, double* g);
+int orc_va13(int n, const double* x0, int ivmax, orc_fg fg, void* ctx, double* xlast,
+             double* flast, double* fbest, int* nfun);
+
+/* mc11a alone (packed factors a, n(n+1)/2 values); z and w may alias. */
+void orc_va13_update(double* a, int n, double* z, double sig, double* w, int* ir, int mk, double eps);
+
+/* Minimizes n poses (26-float DOF_6D_Modes layout in and out) with VA13 on the
+ * FP32 exact adjoint; the state is (angles, position, receptor modes, ligand
+ * modes). */
+int orc_minimize_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                     const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                     const float* dof, size_t n, int ivmax, float* dof_out, double* e_last,
+                     double* e_best, int* nfun);
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,15 @@ def oracle_lib():
                                       C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(OrcParamsC), P,
                                       C.POINTER(BeadsC)]
+        FG = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.POINTER(C.c_double))
+        L.FG = FG
+        L.orc_va13.restype = C.c_int
+        L.orc_va13.argtypes = [C.c_int, P, C.c_int, FG, C.c_void_p, P, P, P, P]
+        L.orc_minimize_f32.restype = C.c_int
+        L.orc_minimize_f32.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                       C.POINTER(IGridC), C.POINTER(NLGridC),
+                                       C.POINTER(OrcParamsC), P, C.c_size_t, C.c_int, P, P, P, P]
         for fn in (L.orc_adjoint_f64, L.orc_adjoint_f32):
             fn.restype = C.c_int
             fn.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
@@ -245,6 +254,45 @@ def adjoint(wl, dofs=None, w=None, pgrad=False, scale=False, f32=False):
     if pgrad:
         ret += (pg,)
     return ret if len(ret) > 1 else out
+
+
+def va13(fg, x0, ivmax=500):
+    """VA13 on a Python objective fg(x) -> (f, g).  Returns (x_last, f_last,
+    f_best, nfun, evaluations) where evaluations lists every (x, f) asked for."""
+    L = oracle_lib()
+    n = len(x0)
+    evals = []
+
+    def cb(_, xp, fp, gp):
+        x = np.ctypeslib.as_array(xp, (n,)).copy()
+        f, g = fg(x)
+        evals.append((x, float(f)))
+        fp[0] = float(f)
+        for i in range(n):
+            gp[i] = float(g[i])
+
+    cfun = L.FG(cb)
+    x = np.ascontiguousarray(x0, np.float64)
+    xl = np.zeros(n)
+    fl, fb, nf = C.c_double(), C.c_double(), C.c_int()
+    L.orc_va13(n, _p(x), ivmax, cfun, None, _p(xl), C.byref(fl), C.byref(fb), C.byref(nf))
+    return xl, fl.value, fb.value, nf.value, evals
+
+
+def minimize(wl, dofs=None, ivmax=500):
+    """VA13 on the FP32 adjoint for each pose: (dofs_last (n, 26) float32,
+    e_last, e_best, nfun)."""
+    L = oracle_lib()
+    d = as_modes_dofs(wl, dofs)
+    n = d.shape[0]
+    out = np.zeros((n, 26), np.float32)
+    el, eb = np.zeros(n), np.zeros(n)
+    nf = np.zeros(n, np.int32)
+    L.orc_minimize_f32(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                       C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                       C.byref(nl_c(wl.rec_nl)), C.byref(params_c(wl.params)), _p(d), n, ivmax,
+                       _p(out), _p(el), _p(eb), _p(nf))
+    return out, el, eb, nf
 
 
 def beads(wl, dof):
