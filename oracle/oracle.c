@@ -59,7 +59,8 @@ int orc_adjoint_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* i
  * reference builds its DOF_6D<float> requests, and scored by the FP32 exact
  * adjoint (energy and dE/dstate). */
 typedef struct {
-    const ctx_f32* x;
+    const ctx_f32* x32;
+    const ctx_f64* x64;
     unsigned Mr, Ml;
     float dof[26];
     double res[27];
@@ -74,7 +75,8 @@ static void min_fg(void* p, const double* s, double* f, double* g) {
     }
     for (unsigned k = 0; k < m->Mr; ++k) m->dof[6 + k] = (float)s[6 + k];
     for (unsigned k = 0; k < m->Ml; ++k) m->dof[16 + k] = (float)s[6 + m->Mr + k];
-    adj_pose_f32(m->x, 0, 0, m->dof, 1.0, m->res, 0, 0);
+    if (m->x32) adj_pose_f32(m->x32, 0, 0, m->dof, 1.0, m->res, 0, 0);
+    else adj_pose_f64(m->x64, 0, 0, m->dof, 1.0, m->res, 0, 0);
     *f = m->res[0];
     for (int k = 0; k < 3; ++k) {
         g[k] = m->res[4 + k];
@@ -82,6 +84,35 @@ static void min_fg(void* p, const double* s, double* f, double* g) {
     }
     for (unsigned k = 0; k < m->Mr; ++k) g[6 + k] = m->res[7 + k];
     for (unsigned k = 0; k < m->Ml; ++k) g[6 + m->Mr + k] = m->res[17 + k];
+}
+
+static void minimize_poses(min_ctx* m, const float* dof, size_t n, int ivmax, float* dof_out,
+                           double* e_last, double* e_best, int* nfun) {
+    const int dim = 6 + (int)(m->Mr + m->Ml);
+    double s0[26], sl[26];
+    for (size_t p = 0; p < n; ++p) {
+        const float* d = dof + 26 * p;
+        for (int k = 0; k < 3; ++k) {
+            s0[k] = d[3 + k];
+            s0[3 + k] = d[k];
+        }
+        for (unsigned k = 0; k < m->Mr; ++k) s0[6 + k] = d[6 + k];
+        for (unsigned k = 0; k < m->Ml; ++k) s0[6 + m->Mr + k] = d[16 + k];
+        double fl, fb;
+        int nf;
+        orc_va13(dim, s0, ivmax, min_fg, m, sl, &fl, &fb, &nf);
+        float* o = dof_out + 26 * p;
+        memset(o, 0, sizeof(float) * 26);
+        for (int k = 0; k < 3; ++k) {
+            o[3 + k] = (float)sl[k];
+            o[k] = (float)sl[3 + k];
+        }
+        for (unsigned k = 0; k < m->Mr; ++k) o[6 + k] = (float)sl[6 + k];
+        for (unsigned k = 0; k < m->Ml; ++k) o[16 + k] = (float)sl[6 + m->Mr + k];
+        if (e_last) e_last[p] = fl;
+        if (e_best) e_best[p] = fb;
+        if (nfun) nfun[p] = nf;
+    }
 }
 
 int orc_minimize_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
@@ -95,35 +126,30 @@ int orc_minimize_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* 
     ctx_init_f32(&x);
     min_ctx m;
     memset(&m, 0, sizeof(m));
-    m.x = &x;
+    m.x32 = &x;
     m.Mr = rec->M;
     m.Ml = lig->M;
-    const int dim = 6 + (int)(m.Mr + m.Ml);
-    double s0[26], sl[26];
-    for (size_t p = 0; p < n; ++p) {
-        const float* d = dof + 26 * p;
-        for (int k = 0; k < 3; ++k) {
-            s0[k] = d[3 + k];
-            s0[3 + k] = d[k];
-        }
-        for (unsigned k = 0; k < m.Mr; ++k) s0[6 + k] = d[6 + k];
-        for (unsigned k = 0; k < m.Ml; ++k) s0[6 + m.Mr + k] = d[16 + k];
-        double fl, fb;
-        int nf;
-        orc_va13(dim, s0, ivmax, min_fg, &m, sl, &fl, &fb, &nf);
-        float* o = dof_out + 26 * p;
-        memset(o, 0, sizeof(float) * 26);
-        for (int k = 0; k < 3; ++k) {
-            o[3 + k] = (float)sl[k];
-            o[k] = (float)sl[3 + k];
-        }
-        for (unsigned k = 0; k < m.Mr; ++k) o[6 + k] = (float)sl[6 + k];
-        for (unsigned k = 0; k < m.Ml; ++k) o[16 + k] = (float)sl[6 + m.Mr + k];
-        if (e_last) e_last[p] = fl;
-        if (e_best) e_best[p] = fb;
-        if (nfun) nfun[p] = nf;
-    }
+    minimize_poses(&m, dof, n, ivmax, dof_out, e_last, e_best, nfun);
     ctx_free_f32(&x);
+    return 0;
+}
+
+int orc_minimize_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                     const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                     const float* dof, size_t n, int ivmax, float* dof_out, double* e_last,
+                     double* e_best, int* nfun) {
+    ctx_f64 x;
+    memset(&x, 0, sizeof(x));
+    x.rec = rec; x.lig = lig; x.par = par;
+    x.inner = inner; x.outer = outer; x.nl = nl;
+    ctx_init_f64(&x);
+    min_ctx m;
+    memset(&m, 0, sizeof(m));
+    m.x64 = &x;
+    m.Mr = rec->M;
+    m.Ml = lig->M;
+    minimize_poses(&m, dof, n, ivmax, dof_out, e_last, e_best, nfun);
+    ctx_free_f64(&x);
     return 0;
 }
 

@@ -147,6 +147,11 @@ int orc_minimize_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* 
                      const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
                      const float* dof, size_t n, int ivmax, float* dof_out, double* e_last,
                      double* e_best, int* nfun);
+/* The same with the FP64 adjoint (requests still rounded to float DOFs). */
+int orc_minimize_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                     const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                     const float* dof, size_t n, int ivmax, float* dof_out, double* e_last,
+                     double* e_best, int* nfun);
 
 #ifdef __cplusplus
 }

@@ -114,10 +114,11 @@ def oracle_lib():
         L.FG = FG
         L.orc_va13.restype = C.c_int
         L.orc_va13.argtypes = [C.c_int, P, C.c_int, FG, C.c_void_p, P, P, P, P]
-        L.orc_minimize_f32.restype = C.c_int
-        L.orc_minimize_f32.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
-                                       C.POINTER(IGridC), C.POINTER(NLGridC),
-                                       C.POINTER(OrcParamsC), P, C.c_size_t, C.c_int, P, P, P, P]
+        for fn in (L.orc_minimize_f32, L.orc_minimize_f64):
+            fn.restype = C.c_int
+            fn.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                           C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(OrcParamsC), P,
+                           C.c_size_t, C.c_int, P, P, P, P]
         for fn in (L.orc_adjoint_f64, L.orc_adjoint_f32):
             fn.restype = C.c_int
             fn.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
@@ -279,16 +280,16 @@ def va13(fg, x0, ivmax=500):
     return xl, fl.value, fb.value, nf.value, evals
 
 
-def minimize(wl, dofs=None, ivmax=500):
-    """VA13 on the FP32 adjoint for each pose: (dofs_last (n, 26) float32,
-    e_last, e_best, nfun)."""
+def minimize(wl, dofs=None, ivmax=500, f64=False):
+    """VA13 on the FP32 (or, with f64=True, FP64) adjoint for each pose:
+    (dofs_last (n, 26) float32, e_last, e_best, nfun)."""
     L = oracle_lib()
     d = as_modes_dofs(wl, dofs)
     n = d.shape[0]
     out = np.zeros((n, 26), np.float32)
     el, eb = np.zeros(n), np.zeros(n)
     nf = np.zeros(n, np.int32)
-    L.orc_minimize_f32(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+    (L.orc_minimize_f64 if f64 else L.orc_minimize_f32)(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
                        C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
                        C.byref(nl_c(wl.rec_nl)), C.byref(params_c(wl.params)), _p(d), n, ivmax,
                        _p(out), _p(el), _p(eb), _p(nf))
