@@ -80,8 +80,6 @@ struct att_ctx {
     size_t d_w_cap = 0;
     void* d_g = nullptr;
     size_t d_g_cap = 0;
-    void* d_min = nullptr;  // minimizer workspace
-    size_t d_min_cap = 0;
 };
 
 namespace {
@@ -840,6 +838,13 @@ att_status att_event_elapsed_ms(att_ctx* c, void* a, void* b, float* ms) {
     return ATT_OK;
 }
 
+att_status att_device_memory(att_ctx* c, size_t* free_bytes, size_t* total_bytes) {
+    if (!c || !free_bytes || !total_bytes) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ATT_HIP(c, hipMemGetInfo(free_bytes, total_bytes));
+    return ATT_OK;
+}
+
 att_status att_stream_create(att_ctx* c, void** stream) {
     if (!c || !stream) return fail(c, ATT_ERR_INVALID, "null argument");
     ATT_HIP(c, hipSetDevice(c->device));
@@ -1144,16 +1149,55 @@ att_status att_batch_run(att_ctx* c, att_batch* b, void* stream) {
 }  // extern "C"
 
 // ---------------------------------------------------------------- minimizer
+// A minimization job owns its device state (VA13 states, request lists and
+// points, scores) and its scoring descriptor; rounds are issued on a stream
+// without host synchronization, the number of running poses can be copied to
+// host memory asynchronously.  att_minimize_dev runs one job to completion;
+// the serving engine interleaves many.
+struct att_minjob {
+    MinRun R{};
+    PairDesc P{};
+    bool modes = false;
+    size_t dyn = 0;
+    unsigned blocks = 1;
+    void* mem = nullptr;
+    size_t bytes = 0;
+    unsigned* list[2] = {nullptr, nullptr};
+    unsigned* count[2] = {nullptr, nullptr};
+    float* req[2] = {nullptr, nullptr};
+    float* res = nullptr;
+    int cur = 0;
+    int rounds = 0;
+};
+
+namespace {
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+void minjob_layout(size_t n, int dim, unsigned ds, unsigned os, size_t* parts) {
+    parts[0] = al256(sizeof(double) * n * va_elements(dim));
+    parts[1] = al256(sizeof(VaState) * n);
+    parts[2] = al256(sizeof(unsigned) * n);
+    parts[3] = al256(2 * sizeof(unsigned));
+    parts[4] = al256(sizeof(float) * n * ds);
+    parts[5] = al256(sizeof(float) * n * os);
+}
+}  // namespace
+
 extern "C" {
 
-att_status att_minimize_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
-                            const float* d_dofs, int modes, size_t n, int maxFunEval,
-                            float* d_dofs_out, float* d_energy, float* d_ebest, int* d_nfun,
-                            void* stream) {
+size_t att_minjob_bytes(size_t n, int dim, int modes) {
+    size_t p[6];
+    minjob_layout(n, dim, modes ? 26 : 6, modes ? 27 : 7, p);
+    return p[0] + p[1] + 2 * p[2] + p[3] + 2 * p[4] + p[5];
+}
+
+att_status att_minjob_create(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                             int modes, size_t n, int maxFunEval, float* d_dofs_out,
+                             float* d_energy, float* d_ebest, int* d_nfun, att_minjob** out) {
     if (!c) return ATT_ERR_INVALID;
-    if ((n && (!d_dofs || !d_dofs_out || !d_energy)) || maxFunEval < 0)
+    if (!out || (n && (!d_dofs_out || !d_energy)) || maxFunEval < 0)
         return fail(c, ATT_ERR_INVALID, "null buffer or negative evaluation limit");
-    if (n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one call");
+    if (n == 0 || n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "pose count out of range");
     ATT_HIP(c, hipSetDevice(c->device));
     ProtObj *rec, *lig;
     GridObj* g;
@@ -1162,15 +1206,13 @@ att_status att_minimize_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle
     if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
     if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
     if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
+    std::unique_ptr<att_minjob> j(new att_minjob);
     unsigned slots = 0;
-    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
+    j->dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
     constexpr int kWaves = 4;
-    if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
+    if (j->dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
-    if (n == 0) return ATT_OK;
-    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
-
-    MinRun R{};
+    MinRun& R = j->R;
     R.npose = (unsigned)n;
     R.Mr = modes ? (int)rec->d.M : 0;
     R.Ml = modes ? (int)lig->d.M : 0;
@@ -1179,60 +1221,128 @@ att_status att_minimize_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle
     R.modes = modes != 0;
     R.ds = modes ? 26 : 6;
     R.os = modes ? 27 : 7;
-    // workspace: states, scalars, two request lists / counts / point sets, results
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t bSt = al(sizeof(double) * n * va_elements(R.dim)), bSc = al(sizeof(VaState) * n),
-                 bList = al(sizeof(unsigned) * n), bCnt = al(2 * sizeof(unsigned)),
-                 bReq = al(sizeof(float) * n * R.ds), bRes = al(sizeof(float) * n * R.os);
-    if ((st = grow(c, &c->d_min, &c->d_min_cap, bSt + bSc + 2 * bList + bCnt + 2 * bReq + bRes)) != ATT_OK)
-        return st;
-    char* base = static_cast<char*>(c->d_min);
+    size_t pt[6];
+    minjob_layout(n, R.dim, R.ds, R.os, pt);
+    j->bytes = pt[0] + pt[1] + 2 * pt[2] + pt[3] + 2 * pt[4] + pt[5];
+    ATT_HIP(c, hipMalloc(&j->mem, j->bytes));
+    char* base = static_cast<char*>(j->mem);
     R.st = reinterpret_cast<double*>(base);
-    R.sc = reinterpret_cast<VaState*>(base + bSt);
-    unsigned* list[2] = {reinterpret_cast<unsigned*>(base + bSt + bSc),
-                         reinterpret_cast<unsigned*>(base + bSt + bSc + bList)};
-    unsigned* cnt = reinterpret_cast<unsigned*>(base + bSt + bSc + 2 * bList);
-    unsigned* count[2] = {cnt, cnt + 1};
-    float* req[2] = {reinterpret_cast<float*>(base + bSt + bSc + 2 * bList + bCnt),
-                     reinterpret_cast<float*>(base + bSt + bSc + 2 * bList + bCnt + bReq)};
-    float* res = reinterpret_cast<float*>(base + bSt + bSc + 2 * bList + bCnt + 2 * bReq);
+    R.sc = reinterpret_cast<VaState*>(base + pt[0]);
+    j->list[0] = reinterpret_cast<unsigned*>(base + pt[0] + pt[1]);
+    j->list[1] = reinterpret_cast<unsigned*>(base + pt[0] + pt[1] + pt[2]);
+    j->count[0] = reinterpret_cast<unsigned*>(base + pt[0] + pt[1] + 2 * pt[2]);
+    j->count[1] = j->count[0] + 1;
+    j->req[0] = reinterpret_cast<float*>(base + pt[0] + pt[1] + 2 * pt[2] + pt[3]);
+    j->req[1] = reinterpret_cast<float*>(base + pt[0] + pt[1] + 2 * pt[2] + pt[3] + pt[4]);
+    j->res = reinterpret_cast<float*>(base + pt[0] + pt[1] + 2 * pt[2] + pt[3] + 2 * pt[4]);
     R.dofsOut = d_dofs_out;
     R.eLast = d_energy;
     R.eBest = d_ebest;
     R.nfun = d_nfun;
-
-    PairDesc P{};
+    PairDesc& P = j->P;
     P.rec = rec->d; P.lig = lig->d;
     P.inner = g->inner; P.outer = g->outer; P.nl = g->nl;
     P.par = p->d;
     P.bin = g->bin; P.bout = g->bout;
-    P.out = res;
+    P.out = j->res;
     P.dofStride = R.ds; P.outStride = R.os;
     P.n = (unsigned)n; P.tabSlots = slots;
+    j->modes = modes != 0;
     const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
-    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
-    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
+    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, j->dyn);
+    j->blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
+    *out = j.release();
+    return ATT_OK;
+}
 
-    R.list = list[0]; R.count = count[0]; R.req = req[0];
-    ATT_HIP(c, launch_min_start(R, d_dofs, s));
-    int cur = 0;
-    // every pose is evaluated at most maxFunEval + 1 times
-    for (int round = 0; round <= maxFunEval + 1; ++round) {
-        P.dofs = req[cur];
-        P.nDev = count[cur];
-        ATT_HIP(c, launch_adjoint(P, modes != 0, nullptr, blocks, kWaves, dyn, s));
-        const int nxt = cur ^ 1;
-        R.list = list[nxt]; R.count = count[nxt]; R.req = req[nxt];
-        ATT_HIP(c, launch_min_step(R, list[cur], count[cur], req[cur], res, s));
-        cur = nxt;
-        if ((round & 7) == 7) {
-            unsigned running = 0;
-            ATT_HIP(c, hipMemcpyAsync(&running, count[cur], sizeof(unsigned), hipMemcpyDeviceToHost, s));
-            ATT_HIP(c, hipStreamSynchronize(s));
-            if (running == 0) break;
-        }
+att_status att_minjob_start(att_ctx* c, att_minjob* j, const float* d_dofs, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!j || !d_dofs) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    j->cur = 0;
+    j->rounds = 0;
+    j->R.list = j->list[0]; j->R.count = j->count[0]; j->R.req = j->req[0];
+    ATT_HIP(c, launch_min_start(j->R, d_dofs, s));
+    return ATT_OK;
+}
+
+att_status att_minjob_rounds(att_ctx* c, att_minjob* j, int rounds, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!j) return fail(c, ATT_ERR_INVALID, "null job");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    for (int r = 0; r < rounds; ++r) {
+        const int cur = j->cur, nxt = cur ^ 1;
+        j->P.dofs = j->req[cur];
+        j->P.nDev = j->count[cur];
+        ATT_HIP(c, launch_adjoint(j->P, j->modes, nullptr, j->blocks, 4, j->dyn, s));
+        j->R.list = j->list[nxt]; j->R.count = j->count[nxt]; j->R.req = j->req[nxt];
+        ATT_HIP(c, launch_min_step(j->R, j->list[cur], j->count[cur], j->req[cur], j->res, s));
+        j->cur = nxt;
+        ++j->rounds;
     }
     return ATT_OK;
+}
+
+int att_minjob_round_limit(const att_minjob* j) { return j ? j->R.ivmax + 2 : 0; }
+
+att_status att_minjob_running(att_ctx* c, att_minjob* j, unsigned* h_count, void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!j || !h_count) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    ATT_HIP(c, hipMemcpyAsync(h_count, j->count[j->cur], sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    return ATT_OK;
+}
+
+att_status att_minjob_destroy(att_ctx* c, att_minjob* j) {
+    if (!c || !j) return ATT_ERR_INVALID;
+    (void)hipSetDevice(c->device);
+    if (j->mem) (void)hipFree(j->mem);
+    delete j;
+    return ATT_OK;
+}
+
+att_status att_minimize_dev(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,
+                            const float* d_dofs, int modes, size_t n, int maxFunEval,
+                            float* d_dofs_out, float* d_energy, float* d_ebest, int* d_nfun,
+                            void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if ((n && (!d_dofs || !d_dofs_out || !d_energy)) || maxFunEval < 0)
+        return fail(c, ATT_ERR_INVALID, "null buffer or negative evaluation limit");
+    if (n == 0) {
+        // still validate the handles
+        ProtObj *rec, *lig;
+        GridObj* g;
+        ParObj* p;
+        att_status st;
+        if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
+        return check_pair(c, rec, lig, g, p);
+    }
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    att_minjob* j = nullptr;
+    att_status st = att_minjob_create(c, hr, hl, hg, hp, modes, n, maxFunEval, d_dofs_out, d_energy,
+                                      d_ebest, d_nfun, &j);
+    if (st != ATT_OK) return st;
+    unsigned* running = nullptr;
+    if (hipHostMalloc(reinterpret_cast<void**>(&running), sizeof(unsigned)) != hipSuccess) {
+        att_minjob_destroy(c, j);
+        return fail(c, ATT_ERR_NOMEM, "pinned host word");
+    }
+    st = att_minjob_start(c, j, d_dofs, s);
+    // every pose is evaluated at most maxFunEval + 1 times
+    const int limit = att_minjob_round_limit(j);
+    while (st == ATT_OK && j->rounds < limit) {
+        st = att_minjob_rounds(c, j, std::min(8, limit - j->rounds), s);
+        if (st == ATT_OK) st = att_minjob_running(c, j, running, s);
+        if (st == ATT_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(c, ATT_ERR_HIP, "synchronize");
+        if (st == ATT_OK && *running == 0) break;
+    }
+    if (st == ATT_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(c, ATT_ERR_HIP, "synchronize");
+    (void)hipHostFree(running);
+    att_minjob_destroy(c, j);
+    return st;
 }
 
 att_status att_minimize(att_ctx* c, att_handle hr, att_handle hl, att_handle hg, att_handle hp,

@@ -209,6 +209,8 @@ att_status att_event_elapsed_ms(att_ctx* ctx, void* ev_start, void* ev_stop, flo
  * pinned (att_host_alloc) for the copy to overlap.  att_event_query returns
  * ATT_OK when the work before the event has finished, ATT_NOT_READY when not. */
 att_status att_stream_create(att_ctx* ctx, void** stream);
+/* Free and total memory of the context's device (hipMemGetInfo). */
+att_status att_device_memory(att_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
 att_status att_stream_destroy(att_ctx* ctx, void* stream);
 att_status att_memcpy_h2d_async(att_ctx* ctx, void* d_dst, const void* src, size_t bytes, void* stream);
 att_status att_memcpy_d2h_async(att_ctx* ctx, void* dst, const void* d_src, size_t bytes, void* stream);
@@ -270,6 +272,25 @@ att_status att_minimize_dev(att_ctx* ctx, att_handle rec, att_handle lig, att_ha
 att_status att_minimize(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
                         att_handle params, const float* dofs, int modes, size_t n, int maxFunEval,
                         float* dofs_out, float* energy, float* ebest, int* nfun, void* stream);
+
+/* Minimization jobs: the pieces of att_minimize_dev for callers that
+ * interleave several (the serving engine).  A job owns att_minjob_bytes of
+ * device memory; start, then issue rounds (one scoring launch and one
+ * minimizer step each, no host synchronization); att_minjob_running copies the
+ * number of running poses to host memory asynchronously (pinned memory lets
+ * the copy overlap); after att_minjob_round_limit rounds every pose has
+ * finished.  Results land in the buffers given at creation. */
+typedef struct att_minjob att_minjob;
+size_t att_minjob_bytes(size_t n, int dim, int modes);
+att_status att_minjob_create(att_ctx* ctx, att_handle rec, att_handle lig, att_handle grid,
+                             att_handle params, int modes, size_t n, int maxFunEval,
+                             float* d_dofs_out, float* d_energy, float* d_ebest, int* d_nfun,
+                             att_minjob** out);
+att_status att_minjob_start(att_ctx* ctx, att_minjob* job, const float* d_dofs, void* stream);
+att_status att_minjob_rounds(att_ctx* ctx, att_minjob* job, int rounds, void* stream);
+int att_minjob_round_limit(const att_minjob* job);
+att_status att_minjob_running(att_ctx* ctx, att_minjob* job, unsigned* h_count, void* stream);
+att_status att_minjob_destroy(att_ctx* ctx, att_minjob* job);
 
 /* Library build identification (arch, flags). */
 const char* att_build_info(void);

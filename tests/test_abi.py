@@ -19,6 +19,7 @@ def declared(header, prefix):
 @pytest.mark.parametrize("header,lib,prefix", [
     ("attract_hip.h", "libattract_hip.so", "att_"),
     ("attract_gen.h", "libattract_gen.so", "attg_"),
+    ("attract_serve.h", "libattract_service.so", "att_server_"),
 ])
 def test_exports(header, lib, prefix):
     names = declared(header, prefix)
