@@ -274,9 +274,13 @@ extern "C" int attg_build_grid(const attg_geom* g, const float* xyz, const unsig
         const double pz = (double)g->oz + (double)iz * g->spacing;
         // Per source type: sum of 1/r'^s (s = 8 or 12), 1/r'^6 and the matching
         // force directions d/(r r' r'^s), d/(r r' r'^6).
-        double acc[13 * 8];
-        double Et[13], Ft[13][3];
-        for (unsigned t = 0; t < TT; ++t) { Et[t] = 0; Ft[t][0] = Ft[t][1] = Ft[t][2] = 0; }
+        thread_local std::vector<double> accV, EtV, FtV;
+        accV.assign((size_t)TT * 8, 0.0);
+        EtV.assign(TT, 0.0);
+        FtV.assign((size_t)TT * 3, 0.0);
+        double* acc = accV.data();
+        double* Et = EtV.data();
+        double (*Ft)[3] = reinterpret_cast<double (*)[3]>(FtV.data());
         if (linear) {
             std::fill(acc, acc + TT * 8, 0.0);
             for (unsigned j : act) {

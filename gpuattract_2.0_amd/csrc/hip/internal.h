@@ -83,4 +83,36 @@ hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned col
 hipError_t launch_basis_octets(const float* b, float4* oct, unsigned nx, unsigned ny, unsigned nz,
                                unsigned nb, hipStream_t s);
 
+// Grid generation (gridgen.hip).  Geometry in double as the points are
+// computed: p = origin + index * spacing.
+struct GenGeom {
+    unsigned nx, ny, nz;
+    double ox, oy, oz, sp;
+};
+// Pair parameters seen from (source type a, probe type t), index a*(T+1)+t.
+struct GenPair {
+    double rc, ac, ipon, rmin2, emin;
+};
+struct GenSrc {
+    const float4* sorted;       // beads with type 1..T, stably sorted by type {x,y,z,0}
+    const unsigned* typeStart;  // T+2: type a occupies [typeStart[a], typeStart[a+1])
+    const float4* act;          // the same beads in index order
+    const unsigned* actType;
+    unsigned nAct;
+    const float4* chg;          // those with charge != 0, index order {x,y,z,q}
+    unsigned nChg;
+};
+hipError_t launch_gen_grid(const GenGeom& g, const GenSrc& s, const GenPair* tab, unsigned T,
+                           int shape, bool linear, double P2, double sqrtP2, int dielec, double fe,
+                           float4* out, hipStream_t st);
+size_t gen_scan_scratch(size_t npts);
+// Counts per point (cnt), then desc = {count, exclusive offset}; the total
+// lands in scratch[ceil(npts/1024)].
+hipError_t launch_gen_nl_count(const GenGeom& g, const float4* beads, const unsigned* idx,
+                               unsigned nb, double nd2, float cullR2, uint32_t* cnt, uint2* desc,
+                               unsigned long long* scratch, hipStream_t st);
+hipError_t launch_gen_nl_fill(const GenGeom& g, const float4* beads, const unsigned* idx,
+                              unsigned nb, double nd2, float cullR2, const uint2* desc,
+                              uint32_t* list, hipStream_t st);
+
 }  // namespace att

@@ -165,6 +165,51 @@ att_status upload_octets(att_ctx* c, Obj* o, const att_grid_desc& g, float4** ou
     return ATT_OK;
 }
 
+// Expands a device-resident point grid into voxel octets (owned by o).
+att_status expand_octets(att_ctx* c, Obj* o, const att_grid_desc& g, const float4* d_points,
+                         float4** out) {
+    const size_t noct = (size_t)(g.nx - 1) * (g.ny - 1) * (g.nz - 1) * g.numGrids * 8;
+    void* oct = nullptr;
+    ATT_HIP(c, hipMalloc(&oct, sizeof(float4) * noct));
+    o->allocs.push_back(oct);
+    ATT_HIP(c, launch_octets(d_points, static_cast<float4*>(oct), g.nx, g.ny, g.nz, g.numGrids,
+                             c->stream));
+    *out = static_cast<float4*>(oct);
+    return ATT_OK;
+}
+
+// Neighbour-list constants and the plateau-ratio table (model/NLGrid.tpp:
+// 44-51) of a grid whose desc/list are already on the device.
+att_status attach_nl(att_ctx* c, GridObj* o, unsigned nx, unsigned ny, unsigned nz, float ox,
+                     float oy, float oz, float spacing, float plateau, const uint2* ddesc,
+                     const uint32_t* dlist) {
+    const float P2 = plateau * plateau;
+    const int size = (int)(10000 * P2);
+    std::vector<float> ratio((size_t)size + 1);
+    for (int k = 0; k <= size; ++k) {
+        const double d2 = ((k + 0.5) / 10000);
+        ratio[k] = (float)std::sqrt(d2 / (double)P2);
+    }
+    float* dratio;
+    att_status st;
+    if ((st = dev_copy(c, o, ratio.data(), ratio.size(), &dratio)) != ATT_OK) return st;
+    DNL& d = o->nl;
+    d.desc = ddesc;
+    d.list = dlist;
+    d.ratio = dratio;
+    d.nx = nx; d.ny = ny; d.nz = nz;
+    const float dv = spacing;
+    d.lox = ox; d.loy = oy; d.loz = oz;
+    d.hix = ox + (float)(nx - 1) * dv;
+    d.hiy = oy + (float)(ny - 1) * dv;
+    d.hiz = oz + (float)(nz - 1) * dv;
+    d.dVox_inv = 1.f / dv;
+    d.P2 = P2;
+    d.P2_inv = 1.f / P2;
+    o->hasNL = true;
+    return ATT_OK;
+}
+
 att_status check_grid_desc(att_ctx* c, const att_grid_desc* g, const char* what) {
     if (!g || !g->data) return fail(c, ATT_ERR_INVALID, std::string(what) + ": missing grid data");
     if (g->nx < 2 || g->ny < 2 || g->nz < 2 || g->numGrids < 1)
@@ -346,30 +391,9 @@ att_status att_upload_grid(att_ctx* c, const att_grid_desc* inner, const att_gri
         uint32_t* dlist = nullptr;
         if ((st = dev_copy(c, o.get(), reinterpret_cast<const uint2*>(nl->desc), nv, &ddesc)) != ATT_OK) return st;
         if (nl->numEl > 0 && (st = dev_copy(c, o.get(), nl->list, nl->numEl, &dlist)) != ATT_OK) return st;
-        // Plateau ratios, model/NLGrid.tpp:44-51.
-        const float P2 = nl->plateau * nl->plateau;
-        const int size = (int)(10000 * P2);
-        std::vector<float> ratio((size_t)size + 1);
-        for (int k = 0; k <= size; ++k) {
-            const double d2 = ((k + 0.5) / 10000);
-            ratio[k] = (float)std::sqrt(d2 / (double)P2);
-        }
-        float* dratio;
-        if ((st = dev_copy(c, o.get(), ratio.data(), ratio.size(), &dratio)) != ATT_OK) return st;
-        DNL& d = o->nl;
-        d.desc = ddesc;
-        d.list = dlist;
-        d.ratio = dratio;
-        d.nx = nl->nx; d.ny = nl->ny; d.nz = nl->nz;
-        const float dv = nl->spacing;
-        d.lox = nl->ox; d.loy = nl->oy; d.loz = nl->oz;
-        d.hix = nl->ox + (float)(nl->nx - 1) * dv;
-        d.hiy = nl->oy + (float)(nl->ny - 1) * dv;
-        d.hiz = nl->oz + (float)(nl->nz - 1) * dv;
-        d.dVox_inv = 1.f / dv;
-        d.P2 = P2;
-        d.P2_inv = 1.f / P2;
-        o->hasNL = true;
+        if ((st = attach_nl(c, o.get(), nl->nx, nl->ny, nl->nz, nl->ox, nl->oy, nl->oz, nl->spacing,
+                            nl->plateau, ddesc, dlist)) != ATT_OK)
+            return st;
     }
     *out = store(c, std::move(o));
     return ATT_OK;
@@ -1370,6 +1394,279 @@ att_status att_minimize(att_ctx* c, att_handle hr, att_handle hl, att_handle hg,
     if (ebest) ATT_HIP(c, hipMemcpyAsync(ebest, d_eb, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     if (nfun) ATT_HIP(c, hipMemcpyAsync(nfun, d_nf, sizeof(int) * n, hipMemcpyDeviceToHost, s));
     ATT_HIP(c, hipStreamSynchronize(s));
+    return ATT_OK;
+}
+
+}  // extern "C"
+
+// --- grid generation on the device -----------------------------------------
+
+namespace {
+
+// Device copies of a body's beads and of the pair table, arranged for the
+// generator kernels (gridgen.hip).
+struct GenInputs {
+    Tmp tmp;
+    GenSrc src{};
+    float4* nlBeads = nullptr;
+    unsigned* nlIdx = nullptr;
+    unsigned nNl = 0;
+    GenPair* tab = nullptr;
+    bool linear = true;
+    double P2 = 0.0;
+};
+
+att_status check_body(att_ctx* c, const att_protein_desc* b) {
+    if (!b || !b->xyz || !b->type || !b->charge)
+        return fail(c, ATT_ERR_INVALID, "body: missing coordinates, types or charges");
+    if (b->n == 0) return fail(c, ATT_ERR_INVALID, "body has no beads");
+    return ATT_OK;
+}
+
+att_status check_geom(att_ctx* c, const att_grid_geom* g, unsigned numGrids, const char* what) {
+    if (!g) return fail(c, ATT_ERR_INVALID, std::string(what) + ": missing geometry");
+    if (g->nx < 2 || g->ny < 2 || g->nz < 2)
+        return fail(c, ATT_ERR_INVALID, std::string(what) + ": grid needs >= 2 points per axis");
+    if (!(g->spacing > 0.f) || !std::isfinite(g->spacing) || !std::isfinite(g->ox) ||
+        !std::isfinite(g->oy) || !std::isfinite(g->oz))
+        return fail(c, ATT_ERR_INVALID, std::string(what) + ": spacing must be positive, origin finite");
+    if ((uint64_t)g->nx * g->ny * g->nz * numGrids * 8 >= (1ull << 34))
+        return fail(c, ATT_ERR_UNSUPPORTED, std::string(what) + ": grid exceeds 2^31 points");
+    return ATT_OK;
+}
+
+GenGeom gen_geom(const att_grid_geom& g) {
+    return GenGeom{g.nx, g.ny, g.nz, (double)g.ox, (double)g.oy, (double)g.oz, (double)g.spacing};
+}
+
+att_grid_desc grid_of(const att_grid_geom& g, unsigned numGrids) {
+    att_grid_desc d{};
+    d.nx = g.nx; d.ny = g.ny; d.nz = g.nz;
+    d.numGrids = numGrids;
+    d.ox = g.ox; d.oy = g.oy; d.oz = g.oz;
+    d.spacing = g.spacing;
+    return d;
+}
+
+// Beads with type 1..T feed the potential grids (sorted stably by type for the
+// per-type sums, and in index order), the charged ones among them grid 0;
+// beads with type != 0 feed the neighbour lists.  The table is seen from
+// (source type a, probe type t): a receptor's probes are ligand types,
+// table[(a-1)*T + (t-1)]; a ligand's probes are receptor types,
+// table[(t-1)*T + (a-1)].
+att_status gen_prepare(att_ctx* c, const att_protein_desc* b, const att_pair_params* table,
+                       unsigned T, int srcIsLigand, float plateau, GenInputs& in) {
+    const unsigned n = b->n;
+    const float* X = b->xyz;
+    const float* Y = b->xyz + n;
+    const float* Z = b->xyz + 2 * (size_t)n;
+    std::vector<float4> act, chg, nlb;
+    std::vector<unsigned> actType, nlIdx;
+    std::vector<unsigned> typeStart(T + 2, 0);
+    for (unsigned j = 0; j < n; ++j) {
+        const unsigned t = b->type[j];
+        if (t != 0) {
+            nlb.push_back(make_float4(X[j], Y[j], Z[j], 0.f));
+            nlIdx.push_back(j);
+        }
+        if (T == 0 || t == 0 || t > T) continue;
+        act.push_back(make_float4(X[j], Y[j], Z[j], 0.f));
+        actType.push_back(t);
+        typeStart[t + 1]++;
+        if (b->charge[j] != 0.f) chg.push_back(make_float4(X[j], Y[j], Z[j], b->charge[j]));
+    }
+    for (unsigned a = 1; a <= T; ++a) typeStart[a + 1] += typeStart[a];
+    std::vector<float4> sorted(act.size());
+    {
+        std::vector<unsigned> pos(typeStart.begin(), typeStart.end());
+        for (size_t k = 0; k < act.size(); ++k) sorted[pos[actType[k]]++] = act[k];
+    }
+    in.P2 = (double)plateau * (double)plateau;
+    std::vector<GenPair> tab((size_t)(T + 1) * (T + 1), GenPair{0, 0, 0, 0, 0});
+    in.linear = true;
+    for (unsigned a = 1; a <= T; ++a)
+        for (unsigned t = 1; t <= T; ++t) {
+            const att_pair_params& p = table[srcIsLigand ? (t - 1) * T + (a - 1) : (a - 1) * T + (t - 1)];
+            tab[(size_t)a * (T + 1) + t] =
+                GenPair{(double)p.rc, (double)p.ac, (double)p.ipon, (double)p.rmin2, (double)p.emin};
+            if ((double)p.rmin2 > in.P2) in.linear = false;
+        }
+    auto up = [&](auto** dst, const auto& v) -> hipError_t {
+        hipError_t e = in.tmp.alloc(dst, v.size());
+        if (e == hipSuccess && !v.empty())
+            e = hipMemcpy(*dst, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice);
+        return e;
+    };
+    float4 *dsorted, *dact, *dchg;
+    unsigned *dstart, *dtype;
+    ATT_HIP(c, up(&dsorted, sorted));
+    ATT_HIP(c, up(&dstart, typeStart));
+    ATT_HIP(c, up(&dact, act));
+    ATT_HIP(c, up(&dtype, actType));
+    ATT_HIP(c, up(&dchg, chg));
+    ATT_HIP(c, up(&in.nlBeads, nlb));
+    ATT_HIP(c, up(&in.nlIdx, nlIdx));
+    ATT_HIP(c, up(&in.tab, tab));
+    in.src = GenSrc{dsorted, dstart, dact, dtype, (unsigned)act.size(), dchg, (unsigned)chg.size()};
+    in.nNl = (unsigned)nlb.size();
+    return ATT_OK;
+}
+
+att_status gen_grid(att_ctx* c, const GenInputs& in, const att_grid_geom& g, unsigned T, int shape,
+                    int dielec, float ffelec, float4* d_out) {
+    ATT_HIP(c, launch_gen_grid(gen_geom(g), in.src, in.tab, T, shape, in.linear, in.P2,
+                               std::sqrt(in.P2), dielec, (double)ffelec, d_out, c->stream));
+    return ATT_OK;
+}
+
+// Neighbour lists of the points of g: desc (npts uint2) and list (total
+// entries) allocated by `alloc`, which keeps or frees them.
+template <class Alloc>
+att_status gen_nl(att_ctx* c, const GenInputs& in, const att_grid_geom& g, float neighbourdis,
+                  Alloc&& alloc, uint2** d_desc, uint32_t** d_list, uint64_t* total) {
+    const size_t npts = (size_t)g.nx * g.ny * g.nz;
+    const double nd = neighbourdis;
+    const double nd2 = nd * nd;
+    const double cr = nd + 0.01;
+    const float cullR2 = (float)(cr * cr);
+    Tmp tmp;
+    uint32_t* cnt;
+    unsigned long long* scratch;
+    ATT_HIP(c, tmp.alloc(&cnt, npts));
+    ATT_HIP(c, tmp.alloc(&scratch, gen_scan_scratch(npts) / sizeof(unsigned long long)));
+    ATT_HIP(c, alloc((void**)d_desc, sizeof(uint2) * npts));
+    const GenGeom gg = gen_geom(g);
+    ATT_HIP(c, launch_gen_nl_count(gg, in.nlBeads, in.nlIdx, in.nNl, nd2, cullR2, cnt, *d_desc,
+                                   scratch, c->stream));
+    unsigned long long tot = 0;
+    const size_t nsb = (npts + 1023) / 1024;
+    ATT_HIP(c, hipMemcpyAsync(&tot, scratch + nsb, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    if (tot >= (1ull << 32)) return fail(c, ATT_ERR_UNSUPPORTED, "neighbour list exceeds 2^32 entries");
+    *total = tot;
+    *d_list = nullptr;
+    if (tot > 0) {
+        ATT_HIP(c, alloc((void**)d_list, sizeof(uint32_t) * tot));
+        ATT_HIP(c, launch_gen_nl_fill(gg, in.nlBeads, in.nlIdx, in.nNl, nd2, cullR2, *d_desc,
+                                      *d_list, c->stream));
+    }
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    return ATT_OK;
+}
+
+att_status check_gen_args(att_ctx* c, const att_protein_desc* body, const att_pair_params* table,
+                          unsigned T, int shape, int dielec, float plateau) {
+    att_status st;
+    if ((st = check_body(c, body)) != ATT_OK) return st;
+    if (!table) return fail(c, ATT_ERR_INVALID, "missing pair table");
+    if (T == 0 || T > 99) return fail(c, ATT_ERR_INVALID, "number of types must be 1..99");
+    if (shape != 8 && shape != 12) return fail(c, ATT_ERR_INVALID, "potential shape must be 8 or 12");
+    if (dielec != 0 && dielec != 1) return fail(c, ATT_ERR_INVALID, "dielectric must be 0 (constant) or 1 (variable)");
+    if (!(plateau > 0.f) || !std::isfinite(plateau)) return fail(c, ATT_ERR_INVALID, "plateau must be positive");
+    return ATT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+att_status att_generate_grid(att_ctx* c, const att_protein_desc* body, const att_pair_params* table,
+                             unsigned numTypes, int potShape, int dielec, float ffelec,
+                             float plateau, int srcIsLigand, const att_grid_geom* geom, float* out) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!out) return fail(c, ATT_ERR_INVALID, "null output");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_status st;
+    if ((st = check_gen_args(c, body, table, numTypes, potShape, dielec, plateau)) != ATT_OK) return st;
+    if ((st = check_geom(c, geom, numTypes + 1, "grid")) != ATT_OK) return st;
+    GenInputs in;
+    if ((st = gen_prepare(c, body, table, numTypes, srcIsLigand, plateau, in)) != ATT_OK) return st;
+    const size_t n4 = (size_t)geom->nx * geom->ny * geom->nz * (numTypes + 1);
+    Tmp tmp;
+    float4* d;
+    ATT_HIP(c, tmp.alloc(&d, n4));
+    if ((st = gen_grid(c, in, *geom, numTypes, potShape, dielec, ffelec, d)) != ATT_OK) return st;
+    ATT_HIP(c, hipMemcpyAsync(out, d, sizeof(float4) * n4, hipMemcpyDeviceToHost, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    return ATT_OK;
+}
+
+att_status att_generate_nl(att_ctx* c, const att_protein_desc* body, const att_grid_geom* geom,
+                           float neighbourdis, uint32_t* desc, uint32_t* list,
+                           size_t listCapacity, uint64_t* total) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!desc || !total) return fail(c, ATT_ERR_INVALID, "null output");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_status st;
+    if ((st = check_body(c, body)) != ATT_OK) return st;
+    if ((st = check_geom(c, geom, 1, "neighbour grid")) != ATT_OK) return st;
+    if (!(neighbourdis > 0.f) || !std::isfinite(neighbourdis))
+        return fail(c, ATT_ERR_INVALID, "neighbour distance must be positive");
+    GenInputs in;
+    if ((st = gen_prepare(c, body, nullptr, 0, 0, 1.f, in)) != ATT_OK) return st;
+    Tmp tmp;
+    auto alloc = [&](void** p, size_t bytes) { return tmp.alloc(reinterpret_cast<char**>(p), bytes); };
+    uint2* d_desc;
+    uint32_t* d_list;
+    uint64_t tot = 0;
+    if ((st = gen_nl(c, in, *geom, neighbourdis, alloc, &d_desc, &d_list, &tot)) != ATT_OK) return st;
+    *total = tot;
+    const size_t npts = (size_t)geom->nx * geom->ny * geom->nz;
+    ATT_HIP(c, hipMemcpy(desc, d_desc, sizeof(uint2) * npts, hipMemcpyDeviceToHost));
+    if (list && tot > 0) {
+        if (listCapacity < tot) return fail(c, ATT_ERR_INVALID, "list capacity below the total");
+        ATT_HIP(c, hipMemcpy(list, d_list, sizeof(uint32_t) * tot, hipMemcpyDeviceToHost));
+    }
+    return ATT_OK;
+}
+
+att_status att_build_grid(att_ctx* c, const att_protein_desc* body, const att_pair_params* table,
+                          unsigned numTypes, int potShape, int dielec, float ffelec, float plateau,
+                          int srcIsLigand, const att_grid_geom* inner, const att_grid_geom* outer,
+                          float neighbourdis, att_handle* out) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!out) return fail(c, ATT_ERR_INVALID, "null output");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_status st;
+    if ((st = check_gen_args(c, body, table, numTypes, potShape, dielec, plateau)) != ATT_OK) return st;
+    if ((st = check_geom(c, inner, numTypes + 1, "inner")) != ATT_OK) return st;
+    if ((st = check_geom(c, outer, numTypes + 1, "outer")) != ATT_OK) return st;
+    if (neighbourdis > 0.f && !std::isfinite(neighbourdis))
+        return fail(c, ATT_ERR_INVALID, "neighbour distance must be finite");
+    GenInputs in;
+    if ((st = gen_prepare(c, body, table, numTypes, srcIsLigand, plateau, in)) != ATT_OK) return st;
+    auto o = std::make_unique<GridObj>();
+    const att_grid_geom* geoms[2] = {inner, outer};
+    float4* oct[2];
+    for (int k = 0; k < 2; ++k) {
+        const att_grid_desc gd = grid_of(*geoms[k], numTypes + 1);
+        Tmp tmp;
+        float4* d;
+        ATT_HIP(c, tmp.alloc(&d, (size_t)gd.nx * gd.ny * gd.nz * gd.numGrids));
+        if ((st = gen_grid(c, in, *geoms[k], numTypes, potShape, dielec, ffelec, d)) != ATT_OK) return st;
+        if ((st = expand_octets(c, o.get(), gd, d, &oct[k])) != ATT_OK) return st;
+        ATT_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    o->inner = make_dgrid(grid_of(*inner, numTypes + 1), oct[0]);
+    o->outer = make_dgrid(grid_of(*outer, numTypes + 1), oct[1]);
+    if (neighbourdis > 0.f) {
+        Obj* owner = o.get();
+        auto alloc = [&](void** p, size_t bytes) {
+            hipError_t e = hipMalloc(p, bytes);
+            if (e == hipSuccess) owner->allocs.push_back(*p);
+            return e;
+        };
+        uint2* d_desc;
+        uint32_t* d_list;
+        uint64_t tot = 0;
+        if ((st = gen_nl(c, in, *inner, neighbourdis, alloc, &d_desc, &d_list, &tot)) != ATT_OK) return st;
+        o->nlEmpty = tot == 0;
+        o->nlMaxIndex = tot ? body->n - 1 : 0;  // bound: the list holds indices of this body
+        if ((st = attach_nl(c, o.get(), inner->nx, inner->ny, inner->nz, inner->ox, inner->oy,
+                            inner->oz, inner->spacing, plateau, d_desc, d_list)) != ATT_OK)
+            return st;
+    }
+    *out = store(c, std::move(o));
     return ATT_OK;
 }
 

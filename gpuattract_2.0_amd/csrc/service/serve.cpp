@@ -49,6 +49,11 @@ att_status set_err(att_status st, const std::string& msg) {
 struct Receptor {
     as::ProteinData prot;
     as::GridUnionData grid;
+    // a receptor registered as a body: grids built on the device on first use
+    bool fromBody = false;
+    uint64_t par = 0;
+    att_grid_geom inner{}, outer{};
+    float plateau = 0.f, nd = 0.f;
     size_t bytes = 0;
     att_handle hProt = 0, hGrid = 0;
     bool resident = false;
@@ -116,6 +121,24 @@ size_t receptor_bytes(const Receptor& r) {
     const size_t lut = (size_t)(10000.0 * (double)nl.plateau * nl.plateau) + 1;
     return protein_bytes(r.prot) + grid_bytes(r.grid.inner) + grid_bytes(r.grid.outer) +
            4 * nl.desc.size() + 4 * nl.list.size() + 4 * lut;
+}
+
+// A receptor whose grids are built on the device: protein, octet grids, the
+// neighbour-list descriptors, a bound on the list (every (bead, point) pair
+// within the neighbour distance counted once), the ratio table, and the
+// transient point grid of the generator.
+size_t body_receptor_bytes(const Receptor& r, unsigned T) {
+    auto octets = [&](const att_grid_geom& g) {
+        return (size_t)(T + 1) * (g.nx - 1) * (g.ny - 1) * (g.nz - 1) * 128;
+    };
+    auto points = [&](const att_grid_geom& g) { return (size_t)(T + 1) * g.nx * g.ny * g.nz * 16; };
+    const size_t npts = (size_t)r.inner.nx * r.inner.ny * r.inner.nz;
+    const double sp = r.inner.spacing, nd = r.nd;
+    const double perBead = 4.18879 * nd * nd * nd / (sp * sp * sp) + 1.0;
+    const size_t listBound = std::min<size_t>((size_t)(perBead * r.prot.n), npts * (size_t)r.prot.n);
+    const size_t lut = (size_t)(10000.0 * (double)r.plateau * r.plateau) + 1;
+    return protein_bytes(r.prot) + octets(r.inner) + octets(r.outer) + 12 * npts + 4 * listBound +
+           4 * lut + std::max(points(r.inner), points(r.outer));
 }
 
 void copy_protein(as::ProteinData& p, const att_protein_desc& d) {
@@ -280,10 +303,17 @@ bool att_server::admit(Job* j) {
     };
     if (!r->resident) {
         const att_protein_desc pd = r->prot.desc();
-        const att_grid_desc gi = r->grid.innerDesc(), go = r->grid.outerDesc();
-        const att_nlgrid_desc nl = r->grid.nlDesc();
         if (att_upload_protein(ctx, &pd, &r->hProt) != ATT_OK) return bad("receptor upload");
-        if (att_upload_grid(ctx, &gi, &go, &nl, &r->hGrid) != ATT_OK) {
+        att_status gs;
+        if (r->fromBody) {
+            gs = att_build_grid(ctx, &pd, p->table.data(), p->T, p->shape, p->dielec, p->ffelec,
+                                r->plateau, 0, &r->inner, &r->outer, r->nd, &r->hGrid);
+        } else {
+            const att_grid_desc gi = r->grid.innerDesc(), go = r->grid.outerDesc();
+            const att_nlgrid_desc nl = r->grid.nlDesc();
+            gs = att_upload_grid(ctx, &gi, &go, &nl, &r->hGrid);
+        }
+        if (gs != ATT_OK) {
             att_release(ctx, r->hProt);
             r->hProt = 0;
             return bad("grid upload");
@@ -503,6 +533,30 @@ att_status att_server_add_receptor(att_server* s, const att_protein_desc* rec,
     return ATT_OK;
 }
 
+att_status att_server_add_receptor_body(att_server* s, const att_protein_desc* rec, uint64_t params,
+                                        const att_grid_geom* inner, const att_grid_geom* outer,
+                                        float plateau, float neighbourdis, uint64_t* id) {
+    if (!s || !id || !inner || !outer) return set_err(ATT_ERR_INVALID, "null argument");
+    if (!protein_ok(rec)) return set_err(ATT_ERR_INVALID, "incomplete receptor description");
+    if (!(plateau > 0.f) || !(neighbourdis > 0.f))
+        return set_err(ATT_ERR_INVALID, "plateau and neighbour distance must be positive");
+    std::unique_ptr<Receptor> r(new Receptor);
+    copy_protein(r->prot, *rec);
+    r->fromBody = true;
+    r->par = params;
+    r->inner = *inner;
+    r->outer = *outer;
+    r->plateau = plateau;
+    r->nd = neighbourdis;
+    std::lock_guard<std::mutex> lk(s->m);
+    auto pit = s->params.find(params);
+    if (pit == s->params.end()) return set_err(ATT_ERR_HANDLE, "unknown table id");
+    r->bytes = body_receptor_bytes(*r, pit->second->T);
+    *id = s->nextId++;
+    s->receptors[*id] = std::move(r);
+    return ATT_OK;
+}
+
 att_status att_server_add_ligand(att_server* s, const att_protein_desc* lig, uint64_t* id) {
     if (!s || !id) return set_err(ATT_ERR_INVALID, "null argument");
     if (!protein_ok(lig)) return set_err(ATT_ERR_INVALID, "incomplete ligand description");
@@ -550,6 +604,9 @@ att_status att_server_submit(att_server* s, uint64_t rec, uint64_t lig, uint64_t
         std::lock_guard<std::mutex> lk(s->m);
         if (!s->receptors.count(rec) || !s->ligands.count(lig) || !s->params.count(par))
             return set_err(ATT_ERR_HANDLE, "unknown receptor, ligand or table id");
+        const Receptor& r = *s->receptors[rec];
+        if (r.fromBody && r.par != par)
+            return set_err(ATT_ERR_INVALID, "the receptor's grids were registered with another table");
         if (s->stop) return set_err(ATT_ERR_INVALID, "server is shutting down");
         j->id = s->nextId++;
         *job = j->id;

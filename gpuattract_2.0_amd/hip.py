@@ -39,6 +39,16 @@ class NLDesc(C.Structure):
                 ("plateau", C.c_float), ("desc", P), ("list", P), ("numEl", C.c_size_t)]
 
 
+class GridGeom(C.Structure):
+    _fields_ = [("nx", C.c_uint), ("ny", C.c_uint), ("nz", C.c_uint),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("spacing", C.c_float)]
+
+
+def grid_geom(ge):
+    """att_grid_geom of a workload Geom."""
+    return GridGeom(ge.nx, ge.ny, ge.nz, ge.ox, ge.oy, ge.oz, ge.spacing)
+
+
 class BatchPair(C.Structure):
     _fields_ = [("rec", C.c_uint64), ("lig", C.c_uint64), ("grid", C.c_uint64),
                 ("params", C.c_uint64), ("d_dofs", P), ("d_out", P), ("d_w", P),
@@ -140,6 +150,13 @@ def lib():
         L.att_minimize.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, C.c_int, P, P, P, P, P]
         L.att_minimize_dev.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, C.c_int, P, P, P, P,
                                        P]
+        L.att_generate_grid.argtypes = [P, C.POINTER(ProteinDesc), P, C.c_uint, C.c_int, C.c_int,
+                                        C.c_float, C.c_float, C.c_int, C.POINTER(GridGeom), P]
+        L.att_generate_nl.argtypes = [P, C.POINTER(ProteinDesc), C.POINTER(GridGeom), C.c_float,
+                                      P, P, C.c_size_t, C.POINTER(C.c_uint64)]
+        L.att_build_grid.argtypes = [P, C.POINTER(ProteinDesc), P, C.c_uint, C.c_int, C.c_int,
+                                     C.c_float, C.c_float, C.c_int, C.POINTER(GridGeom),
+                                     C.POINTER(GridGeom), C.c_float, C.POINTER(H)]
         L.att_batch_create.argtypes = [P, C.POINTER(BatchPair), C.c_uint, C.POINTER(P)]
         L.att_batch_destroy.argtypes = [P, P]
         L.att_batch_buckets.argtypes = [P]
@@ -262,6 +279,40 @@ class Context:
 
     def release(self, handle):
         self._chk(self.L.att_release(self.h, handle))
+
+    # -- grid generation on the device -----------------------------------
+    def generate_grid(self, body, p, geom, plateau=10.0, src_is_ligand=False):
+        """The (T+1) point grids of `body` over `geom` (a workload Geom), as a
+        flat float32 array [grid][z][y][x][4]."""
+        d, g, tab = protein_desc(body), grid_geom(geom), param_table(p)
+        out = np.zeros((p.T + 1) * geom.nvox * 4, np.float32)
+        self._chk(self.L.att_generate_grid(self.h, C.byref(d), _p(tab), p.T, p.shape, p.dielec,
+                                           p.ffelec, plateau, 1 if src_is_ligand else 0,
+                                           C.byref(g), _p(out)))
+        return out
+
+    def generate_nl(self, body, geom, neighbourdis=10.8):
+        """(desc, list) of the neighbour lists of `body` over `geom`."""
+        d, g = protein_desc(body), grid_geom(geom)
+        desc = np.zeros(geom.nvox * 2, np.uint32)
+        tot = C.c_uint64(0)
+        self._chk(self.L.att_generate_nl(self.h, C.byref(d), C.byref(g), neighbourdis, _p(desc),
+                                         None, 0, C.byref(tot)))
+        lst = np.zeros(max(1, tot.value), np.uint32)
+        self._chk(self.L.att_generate_nl(self.h, C.byref(d), C.byref(g), neighbourdis, _p(desc),
+                                         _p(lst), len(lst), C.byref(tot)))
+        return desc, lst[:tot.value]
+
+    def build_grid(self, body, p, inner, outer, plateau=10.0, neighbourdis=10.8,
+                   src_is_ligand=False):
+        """Grid handle built on the device (neighbourdis <= 0: no neighbour list)."""
+        d, tab = protein_desc(body), param_table(p)
+        gi, go = grid_geom(inner), grid_geom(outer)
+        out = H()
+        self._chk(self.L.att_build_grid(self.h, C.byref(d), _p(tab), p.T, p.shape, p.dielec,
+                                        p.ffelec, plateau, 1 if src_is_ligand else 0, C.byref(gi),
+                                        C.byref(go), neighbourdis, C.byref(out)))
+        return out.value
 
     def upload_workload(self, wl):
         """Upload everything a workload needs; returns a dict of handles."""

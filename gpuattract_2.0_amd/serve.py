@@ -49,6 +49,10 @@ def lib():
         L.att_server_add_receptor.argtypes = [P, C.POINTER(hip.ProteinDesc),
                                               C.POINTER(hip.GridDesc), C.POINTER(hip.GridDesc),
                                               C.POINTER(hip.NLDesc), C.POINTER(H)]
+        L.att_server_add_receptor_body.argtypes = [P, C.POINTER(hip.ProteinDesc), H,
+                                                   C.POINTER(hip.GridGeom),
+                                                   C.POINTER(hip.GridGeom), C.c_float, C.c_float,
+                                                   C.POINTER(H)]
         L.att_server_add_ligand.argtypes = [P, C.POINTER(hip.ProteinDesc), C.POINTER(H)]
         L.att_server_add_params.argtypes = [P, P, C.c_uint, C.c_int, C.c_int, C.c_float,
                                             C.c_float, C.POINTER(H)]
@@ -95,6 +99,16 @@ class Server:
                                                  C.byref(nd), C.byref(out)))
         return out.value
 
+    def add_receptor_body(self, body, params_id, inner, outer, plateau=10.0, neighbourdis=10.8):
+        """A receptor whose grids the server builds on the device at first use."""
+        d = hip.protein_desc(body)
+        gi, go = hip.grid_geom(inner), hip.grid_geom(outer)
+        out = H()
+        self._chk(self.L.att_server_add_receptor_body(self.h, C.byref(d), params_id, C.byref(gi),
+                                                      C.byref(go), plateau, neighbourdis,
+                                                      C.byref(out)))
+        return out.value
+
     def add_ligand(self, body):
         d = hip.protein_desc(body)
         out = H()
@@ -108,10 +122,16 @@ class Server:
                                                p.epsilon, p.ffelec, C.byref(out)))
         return out.value
 
-    def add_workload(self, wl):
-        """Registers a workload's receptor (with its grids), ligand and table."""
-        return dict(rec=self.add_receptor(wl.rec, wl.rec_inner, wl.rec_outer, wl.rec_nl),
-                    lig=self.add_ligand(wl.lig), params=self.add_params(wl.params))
+    def add_workload(self, wl, device_grids=False):
+        """Registers a workload's receptor (with its host grids, or as a body
+        whose grids are built on the device), ligand and table."""
+        par = self.add_params(wl.params)
+        if device_grids:
+            rec = self.add_receptor_body(wl.rec, par, wl.rec_inner.geom, wl.rec_outer.geom,
+                                         wl.rec_nl.plateau, wl.meta["neighbourdis"])
+        else:
+            rec = self.add_receptor(wl.rec, wl.rec_inner, wl.rec_outer, wl.rec_nl)
+        return dict(rec=rec, lig=self.add_ligand(wl.lig), params=par)
 
     def submit(self, ids, dofs, max_fun_eval=500):
         d = np.ascontiguousarray(dofs, np.float32)

@@ -383,6 +383,80 @@ def make_batch(seed, n_pairs, rec_range=(500, 5000), lig_range=(100, 1500), mode
     return pairs
 
 
+@dataclasses.dataclass
+class ServeReceptor:
+    body: Body
+    radius: float
+    inner: Geom
+    outer: Geom
+
+
+@dataclasses.dataclass
+class ServeJob:
+    receptor: int          # index into the stream's receptors
+    lig: Body
+    dofs: np.ndarray       # (n, 6) rigid or (n, 26) with modes
+    max_fun_eval: int
+
+
+def make_serve_stream(seed, n_jobs, reuse=0.3, rec_range=(500, 8000), lig_range=(100, 2000),
+                      modes_range=(0, 10), poses_range=(500, 20000), max_fun_eval=200,
+                      outer_spacing=2.6):
+    """Docking jobs of a serving stream: each job docks a new ligand (beads
+    log-uniform in lig_range, modes uniform in modes_range) with poses
+    log-uniform in poses_range against a receptor that is, with probability
+    `reuse`, one of the earlier jobs' receptors and otherwise a new one (beads
+    log-uniform in rec_range, its own modes).  One force field for the stream.
+    Grids are not built: each receptor carries its inner geometry (bounding box
+    + 10 A) and an outer cube that covers the largest ligand on the pose shell.
+    Returns (params, receptors, jobs)."""
+    L = gen_lib()
+
+    def u(entity, k):
+        return L.attg_uniform(seed, entity, k)
+
+    def logu(entity, k, lo, hi):
+        return int(round(math.exp(math.log(lo) + u(entity, k) * (math.log(hi) - math.log(lo)))))
+
+    def uni(entity, k, lo, hi):
+        return min(hi, lo + int(u(entity, k) * (hi - lo + 1)))
+
+    params = make_params(seed)
+    r_lig_max = scaled_radius(lig_range[1], LIG_REF)
+    recs, jobs = [], []
+    for k in range(n_jobs):
+        if recs and u(70, 2 * k) < reuse:
+            ri = min(int(u(70, 2 * k + 1) * len(recs)), len(recs) - 1)
+        else:
+            ri = len(recs)
+            n = logu(71, ri, *rec_range)
+            R = scaled_radius(n, DENSITY_REF)
+            body = make_body(seed + 7919 * (ri + 1), 0, n, R)
+            add_modes(body, seed + 7919 * (ri + 1), 50, uni(72, ri, *modes_range))
+            recs.append(ServeReceptor(body, R, inner_geom(body),
+                                      cube_geom((0.0, 0.0, 0.0), R + 2 * r_lig_max + 2.0,
+                                                outer_spacing)))
+        rec = recs[ri]
+        n_l = logu(73, k, *lig_range)
+        R_l = scaled_radius(n_l, LIG_REF)
+        lig = make_body(seed + 104729 * (k + 1), 1, n_l, R_l)
+        M_l = uni(74, k, *modes_range)
+        add_modes(lig, seed + 104729 * (k + 1), 60, M_l)
+        n_p = logu(75, k, *poses_range)
+        d6 = make_poses(seed + 104729 * (k + 1), n_p, 0.75 * (rec.radius + R_l),
+                        1.0 * (rec.radius + R_l))
+        M_r = rec.body.M
+        if M_r or M_l:
+            d = np.zeros((n_p, 26), np.float32)
+            d[:, :6] = d6
+            d[:, 6:6 + M_r] = make_amplitudes(seed + 104729 * (k + 1), n_p, M_r, entity=5)
+            d[:, 16:16 + M_l] = make_amplitudes(seed + 104729 * (k + 1), n_p, M_l, entity=6)
+        else:
+            d = d6
+        jobs.append(ServeJob(ri, lig, d, max_fun_eval))
+    return params, recs, jobs
+
+
 def checksum(*arrays):
     """Order-sensitive 64-bit checksum of raw array bytes (FNV-1a over 8-byte words)."""
     h = np.uint64(0xCBF29CE484222325)

@@ -134,6 +134,41 @@ att_status att_score_modes_dev(att_ctx* ctx, att_handle rec, att_handle lig, att
                                att_handle ligGrid, att_handle params, const att_dof6m* d_dofs,
                                size_t n, att_result6m* d_out, void* stream);
 
+/* --- grid generation on the device ------------------------------------------
+ * No reference counterpart: the reference reads grids made by ATTRACT's tools
+ * (fileIO/grid_orig.cpp:85-301, IntrplGrid::Desc / NLGrid::Desc).  These build
+ * them from a body on the device with the semantics the scoring path assumes:
+ * grid t>0 at point p = sum over beads of type 1..T of the LJ potential of
+ * (bead type, probe t) at r' = max(r, plateau) (model/forcefield.h:28-83),
+ * force channel -dV/dr' along (p - x)/r; grid 0 = Coulomb at r'
+ * (forcefield.h:85-146); neighbour lists = beads with type != 0 within
+ * neighbourdis of p, ascending index.  Sums in double, rounded once to float.
+ * srcIsLigand selects the table index: 0 (receptor grids, probes are ligand
+ * types) table[(src-1)*T + (probe-1)], 1 (ligand grids) table[(probe-1)*T +
+ * (src-1)].  Geometry: point (ix,iy,iz) = origin + index * spacing. */
+typedef struct {
+    unsigned nx, ny, nz;
+    float ox, oy, oz;
+    float spacing;
+} att_grid_geom;
+/* The (numTypes+1) point grids into host memory: float4 [grid][z][y][x]. */
+att_status att_generate_grid(att_ctx* ctx, const att_protein_desc* body,
+                             const att_pair_params* table, unsigned numTypes, int potShape,
+                             int dielec, float ffelec, float plateau, int srcIsLigand,
+                             const att_grid_geom* geom, float* out);
+/* Neighbour lists into host memory: desc {count, start} per point; list may
+ * be NULL to learn *total first. */
+att_status att_generate_nl(att_ctx* ctx, const att_protein_desc* body, const att_grid_geom* geom,
+                           float neighbourdis, uint32_t* desc, uint32_t* list,
+                           size_t listCapacity, uint64_t* total);
+/* Inner and outer grids and (neighbourdis > 0) the neighbour-list grid over
+ * the inner geometry, built and kept on the device: the handle is used like
+ * one from att_upload_grid. */
+att_status att_build_grid(att_ctx* ctx, const att_protein_desc* body, const att_pair_params* table,
+                          unsigned numTypes, int potShape, int dielec, float ffelec, float plateau,
+                          int srcIsLigand, const att_grid_geom* inner, const att_grid_geom* outer,
+                          float neighbourdis, att_handle* out);
+
 /* --- hand-written adjoint -----------------------------------------------------
  * No reference counterpart (the reference has no gradient beyond its force
  * reduction; service/CPUEnergyService6D.tpp:100-195 defines the energy).

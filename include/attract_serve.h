@@ -66,6 +66,13 @@ const char* att_server_last_error(const att_server* s);
 att_status att_server_add_receptor(att_server* s, const att_protein_desc* rec,
                                    const att_grid_desc* inner, const att_grid_desc* outer,
                                    const att_nlgrid_desc* nl, uint64_t* id);
+/* A receptor given as a body only: its grids (att_build_grid semantics, for the
+ * table `params`, neighbour lists over the inner geometry) are built on the
+ * device when a job first needs them and cached like uploaded ones.  Jobs on
+ * it must use the same table. */
+att_status att_server_add_receptor_body(att_server* s, const att_protein_desc* rec, uint64_t params,
+                                        const att_grid_geom* inner, const att_grid_geom* outer,
+                                        float plateau, float neighbourdis, uint64_t* id);
 att_status att_server_add_ligand(att_server* s, const att_protein_desc* lig, uint64_t* id);
 att_status att_server_add_params(att_server* s, const att_pair_params* table, unsigned numTypes,
                                  int potShape, int dielec, float epsilon, float ffelec,

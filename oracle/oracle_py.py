@@ -168,6 +168,13 @@ def ref_lib():
                                          C.POINTER(IGridC), C.POINTER(NLGridC),
                                          C.POINTER(RefTableC), P, C.c_size_t, P, C.c_int,
                                          C.POINTER(C.c_double)]
+        L.ref_gridfile_load.restype = C.c_int
+        L.ref_gridfile_load.argtypes = [C.c_char_p]
+        L.ref_gridfile_info.restype = None
+        L.ref_gridfile_info.argtypes = [P, P, C.POINTER(C.c_uint64)]
+        L.ref_gridfile_copy.restype = None
+        L.ref_gridfile_copy.argtypes = [P, P, P, P]
+        L.ref_gridfile_free.restype = None
         L.ref_read_params.restype = C.c_int
         L.ref_read_params.argtypes = [C.c_uint, C.c_int, P, P, P, P, P, P, P, P]
         L.ref_ratio_lut.restype = C.c_uint
@@ -384,6 +391,26 @@ def ref_score_modes(wl, dofs=None):
                            _p(rec_pot), _p(lig_pot))
     assert rc == 0
     return res, rec_pot, lig_pot
+
+
+def ref_read_gridfile(path):
+    """An ATTRACT grid file through the reference's reader (valid files only:
+    the reference exits on a malformed one).  Returns dims[11], geo[13]
+    (see ref_harness.cpp), inner, outer, desc, list."""
+    L = ref_lib()
+    if L.ref_gridfile_load(os.fsencode(path)) != 0:
+        raise RuntimeError("reference reader failed")
+    dims = np.zeros(11, np.uint32)
+    geo = np.zeros(13, np.float32)
+    ne = C.c_uint64(0)
+    L.ref_gridfile_info(_p(dims), _p(geo), C.byref(ne))
+    inner = np.zeros(int(dims[0]) * int(dims[1]) * int(dims[2]) * int(dims[3]) * 4, np.float32)
+    outer = np.zeros(int(dims[4]) * int(dims[5]) * int(dims[6]) * int(dims[7]) * 4, np.float32)
+    desc = np.zeros(int(dims[8]) * int(dims[9]) * int(dims[10]) * 2, np.uint32)
+    lst = np.zeros(max(1, ne.value), np.uint32)
+    L.ref_gridfile_copy(_p(inner), _p(outer), _p(desc), _p(lst))
+    L.ref_gridfile_free()
+    return dims, geo, inner, outer, desc, lst[:ne.value]
 
 
 def ref_score_mt(wl, dofs=None, nthreads=16):

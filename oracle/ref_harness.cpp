@@ -15,6 +15,9 @@
 //                   with n_rec != n_lig)
 //  param table    : fileIO/readFile.cpp:439-527 on a written attract.par
 //  plateau ratios : model/NLGrid.tpp:44-51
+//  grid files     : fileIO/readFile.cpp:260-422 (createGridFromGridFile over
+//                   attract::Grid::read_std) on a file written by this
+//                   project's att_gridfile_write
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -416,5 +419,50 @@ int ref_score_rigid_mt(const BodyIn* rec, const BodyIn* lig, const GridIn* inner
     }
     return bad ? -1 : 0;
 }
+
+
+// ATTRACT grid file through the reference's reader.  Only valid files may be
+// given: the reference's reader exits the process on a malformed one.
+static std::shared_ptr<GridUnion<float>> g_file_grid;
+
+int ref_gridfile_load(const char* path) {
+    g_file_grid = createGridFromGridFile<float>(std::string(path));
+    return g_file_grid ? 0 : -1;
+}
+
+// dims: inner nx ny nz numGrids, outer nx ny nz numGrids, neighbour nx ny nz;
+// geo: inner x y z spacing, outer x y z spacing, neighbour x y z spacing plateau^2
+void ref_gridfile_info(unsigned* dims, float* geo, uint64_t* numEl) {
+    auto& in = *g_file_grid->inner;
+    auto& out = *g_file_grid->outer;
+    auto& nl = *g_file_grid->NL;
+    const unsigned d[11] = {in.dimN().x, in.dimN().y, in.dimN().z, in.numTypes(),
+                            out.dimN().x, out.dimN().y, out.dimN().z, out.numTypes(),
+                            nl.dimN().x, nl.dimN().y, nl.dimN().z};
+    std::copy(d, d + 11, dims);
+    const float g[13] = {in.pos().x, in.pos().y, in.pos().z, in.dVox(),
+                         out.pos().x, out.pos().y, out.pos().z, out.dVox(),
+                         nl.pos().x, nl.pos().y, nl.pos().z, nl.dVox(), nl.dPlateau2()};
+    std::copy(g, g + 13, geo);
+    *numEl = nl.neighborListSize();
+}
+
+void ref_gridfile_copy(float* inner, float* outer, uint32_t* desc, uint32_t* list) {
+    auto& in = *g_file_grid->inner;
+    auto& out = *g_file_grid->outer;
+    auto& nl = *g_file_grid->NL;
+    const size_t ni = (size_t)in.dimN().x * in.dimN().y * in.dimN().z * in.numTypes();
+    const size_t no = (size_t)out.dimN().x * out.dimN().y * out.dimN().z * out.numTypes();
+    std::memcpy(inner, in.getHostGridPtr(0), 16 * ni);
+    std::memcpy(outer, out.getHostGridPtr(0), 16 * no);
+    const size_t nv = (size_t)nl.dimN().x * nl.dimN().y * nl.dimN().z;
+    for (size_t v = 0; v < nv; ++v) {
+        desc[2 * v] = nl.grid()[v].numEl;
+        desc[2 * v + 1] = nl.grid()[v].idx;
+    }
+    std::memcpy(list, nl.neighborList(), 4 * (size_t)nl.neighborListSize());
+}
+
+void ref_gridfile_free() { g_file_grid.reset(); }
 
 }  // extern "C"

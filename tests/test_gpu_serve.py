@@ -136,3 +136,21 @@ def test_serve_errors_and_timeouts(ctx, S, pairs):
         j = srv.submit(ids, wl.dofs, 10)
         with pytest.raises(hip.AttError, match="budget"):
             srv.wait(j)
+
+
+def test_serve_device_built_grids(ctx, S, pairs):
+    """A receptor registered as a body: its grids are built on the device at
+    first use and equal the host generator's, so results equal att_minimize
+    on the uploaded host grids bit for bit."""
+    from gpuattract_amd import hip
+    wl = pairs[1]
+    ref = alone(ctx, wl, 40)
+    with S.Server(0) as srv:
+        ids = srv.add_workload(wl, device_grids=True)
+        r = srv.wait(srv.submit(ids, wl.dofs, 40))
+        assert same(r, ref) and r[4]["grid_cached"] == 0
+        r2 = srv.wait(srv.submit(ids, wl.dofs, 40))
+        assert same(r2, ref) and r2[4]["grid_cached"] == 1
+        other = srv.add_params(wl.params)
+        with pytest.raises(hip.AttError, match="another table"):
+            srv.submit(dict(ids, params=other), wl.dofs, 10)
