@@ -160,6 +160,10 @@ def main():
     log(f"forward kernel: {fwd_ms:.3f} ms/step, {n / fwd_ms * 1e3:.4g} poses/s")
     dof_ms = rate(dof_step, reps)
     log(f"adjoint, DOF + mode gradients only: {dof_ms:.3f} ms/step, {n / dof_ms * 1e3:.4g} poses/s")
+    ctx.set_pose_order(False)
+    unordered_ms = rate(step, reps)
+    ctx.set_pose_order(True)
+    log(f"adjoint in input pose order: {unordered_ms:.3f} ms/step, {n / unordered_ms * 1e3:.4g} poses/s")
 
     for _ in range(args.warmup):
         step()
@@ -242,6 +246,7 @@ def main():
                      (" and 12x12 pair-parameter" if pgrad else "") +
                      " gradients), device-resident"),
             "adjoint_dof_modes_only_poses_per_s": n / dof_ms * 1e3,
+            "adjoint_input_order_poses_per_s": n / unordered_ms * 1e3,
             "forward_kernel_poses_per_s": n / fwd_ms * 1e3,
             "plain_item100_poses_per_s": plain,
             "adjoint_vs_plain": value / plain,

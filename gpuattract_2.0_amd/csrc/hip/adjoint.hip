@@ -536,8 +536,18 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     if (s_pg)
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
     const unsigned n = P.nDev ? min(P.nDev[0], P.n) : P.n;
-    for (unsigned pose = blockIdx.x; pose < n; pose += gridDim.x)
-        adjoint_pose<MM, WAVES>(P, pose, S, dyn, tab, s_pg);
+    // With a spatial order, workgroup b (on XCD b % 8) walks the (b % 8)-th
+    // eighth of it (the launch then has at least 8 workgroups); otherwise the
+    // poses in input order, strided by the grid.
+    unsigned first = blockIdx.x, end = n, step = gridDim.x;
+    if (P.order) {
+        const unsigned x = blockIdx.x & 7u;
+        step = (gridDim.x - x + 7u) >> 3;
+        first = (unsigned)(((unsigned long long)n * x) >> 3) + (blockIdx.x >> 3);
+        end = (unsigned)(((unsigned long long)n * (x + 1)) >> 3);
+    }
+    for (unsigned i = first; i < end; i += step)
+        adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, dyn, tab, s_pg);
     if (s_pg) {
         __syncthreads();
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x)

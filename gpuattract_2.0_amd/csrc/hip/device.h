@@ -475,6 +475,7 @@ struct PairDesc {
     float* out;          // n * outStride floats
     unsigned dofStride, outStride, n, tabSlots;
     const unsigned* nDev;  // when set, the number of poses is read here at launch (<= n)
+    const unsigned* order; // when set, a spatial permutation of the poses (XCD ranges)
 };
 
 // Dynamic LDS of a pair: pair table | per-bead {count, start} | queue of

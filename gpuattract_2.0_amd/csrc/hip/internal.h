@@ -83,6 +83,12 @@ hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned col
 hipError_t launch_basis_octets(const float* b, float4* oct, unsigned nx, unsigned ny, unsigned nz,
                                unsigned nb, hipStream_t s);
 
+// Spatial pose order (order.hip): scratch bytes for n poses, and the launch
+// writing the permutation to `order`.
+size_t pose_order_scratch(unsigned n);
+hipError_t launch_pose_order(const float* dofs, unsigned stride, unsigned n, void* scratch,
+                             size_t scratchBytes, unsigned* order, hipStream_t s);
+
 // Grid generation (gridgen.hip).  Geometry in double as the points are
 // computed: p = origin + index * spacing.
 struct GenGeom {

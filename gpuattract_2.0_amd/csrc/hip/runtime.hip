@@ -80,6 +80,9 @@ struct att_ctx {
     size_t d_w_cap = 0;
     void* d_g = nullptr;
     size_t d_g_cap = 0;
+    void* d_ord = nullptr;  // spatial pose order: permutation + sort scratch
+    size_t d_ord_cap = 0;
+    bool poseOrder = true;
 };
 
 namespace {
@@ -262,6 +265,12 @@ const char* att_build_info(void) {
     return "gpuattract_2.0_amd: gfx950, -O3 -ffp-contract=off, packed-fp32 disabled";
 }
 
+att_status att_set_pose_order(att_ctx* c, int enabled) {
+    if (!c) return ATT_ERR_INVALID;
+    c->poseOrder = enabled != 0;
+    return ATT_OK;
+}
+
 att_status att_ctx_create(int hip_device, att_ctx** out) {
     if (!out) return ATT_ERR_INVALID;
     *out = nullptr;
@@ -290,6 +299,7 @@ void att_ctx_destroy(att_ctx* c) {
     if (c->d_in) (void)hipFree(c->d_in);
     if (c->d_out) (void)hipFree(c->d_out);
     if (c->d_pg) (void)hipFree(c->d_pg);
+    if (c->d_ord) (void)hipFree(c->d_ord);
     if (c->d_w) (void)hipFree(c->d_w);
     if (c->d_g) (void)hipFree(c->d_g);
     (void)hipStreamDestroy(c->stream);
@@ -615,7 +625,7 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
         return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
     unsigned slots = 0;
     const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, &slots);
-    constexpr int kWaves = 4;  // one 256-thread workgroup per pose
+    constexpr int kWaves = 2;  // one 128-thread workgroup per pose (timed best of 1, 2, 4)
     if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
@@ -638,6 +648,13 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
     P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
     P.n = (unsigned)n; P.tabSlots = slots;
+    if (c->poseOrder && blocks >= 8 && n >= 4 * (size_t)blocks) {
+        const size_t scratch = pose_order_scratch((unsigned)n);
+        if ((st = grow(c, &c->d_ord, &c->d_ord_cap, scratch + sizeof(unsigned) * n)) != ATT_OK) return st;
+        unsigned* order = reinterpret_cast<unsigned*>(static_cast<char*>(c->d_ord) + scratch);
+        ATT_HIP(c, launch_pose_order(d_dofs, P.dofStride, (unsigned)n, c->d_ord, scratch, order, s));
+        P.order = order;
+    }
     ATT_HIP(c, launch_adjoint(P, modes != 0, pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks,
                               kWaves, dyn, s));
     if (pgrad)

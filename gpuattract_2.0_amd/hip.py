@@ -107,6 +107,7 @@ def lib():
                                "(__graft_entry__.build()); there is no CPU fallback")
         L = C.CDLL(LIBPATH)
         L.att_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
+        L.att_set_pose_order.argtypes = [P, C.c_int]
         L.att_ctx_destroy.argtypes = [P]
         L.att_ctx_destroy.restype = None
         L.att_last_error.argtypes = [P]
@@ -254,6 +255,10 @@ class Context:
     def _chk(self, st):
         if st != ATT_OK:
             raise AttError(st, self.L.att_last_error(self.h).decode())
+
+    def set_pose_order(self, enabled):
+        """Spatial pose order of the single-pair adjoint (default on)."""
+        self._chk(self.L.att_set_pose_order(self.h, 1 if enabled else 0))
 
     # -- uploads ---------------------------------------------------------
     def upload_protein(self, body):

@@ -136,3 +136,30 @@ def test_adjoint_vs_reference_forces(ctx, O, wl_adj_rigid):
     print(f"|exact gradient - reference force| / |force| over {keep.sum()} unclamped poses in "
           f"contact: median {np.median(rel):.3f}, p90 {np.percentile(rel, 90):.3f}")
     assert np.median(rel) < 0.5
+
+
+def test_spatial_pose_order_changes_no_pose_result(W):
+    """Poses scored in the spatial order (default) and in input order give the
+    same per-pose results bit for bit; the parameter-gradient sums agree to
+    summation-order rounding (they are accumulated with LDS float atomics
+    inside a workgroup, so even two runs in one order may differ in the last
+    bits)."""
+    from gpuattract_amd import hip
+    # enough poses for the ordered path (at least four per resident workgroup)
+    wl = W.make_workload("order", 61, 600, 150, 6000, 3, 2, with_basis=True)
+    T = wl.params.T
+    ctx = hip.Context(0)
+    try:
+        hs = ctx.upload_workload(wl)
+        ctx.upload_basis(hs["grid"], wl)
+        a, pa = ctx.adjoint(hs, wl.dofs, pgrad=True, T=T)
+        ctx.set_pose_order(False)
+        b, pb = ctx.adjoint(hs, wl.dofs, pgrad=True, T=T)
+        assert np.array_equal(a, b)
+        assert np.allclose(pa, pb, rtol=1e-4, atol=1e-5 * np.abs(pb).max())
+        ctx.set_pose_order(True)
+        c, pc = ctx.adjoint(hs, wl.dofs, pgrad=True, T=T)
+        assert np.array_equal(a, c)
+        assert np.allclose(pa, pc, rtol=1e-5, atol=1e-6 * np.abs(pc).max())
+    finally:
+        ctx.close()

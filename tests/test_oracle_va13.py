@@ -59,3 +59,48 @@ def test_docking_energies_decrease(W, O):
     # the reported DOFs reproduce the reported energy
     e_chk = O.adjoint(wl, d, f32=True)[:, 0]
     assert np.array_equal(e_chk, el)
+
+
+def _unpack(a, n):
+    """H = L D L^T from mc11a's packed factors (per column i: D_i, then
+    L_{i+1..n-1, i})."""
+    L = np.eye(n)
+    D = np.zeros(n)
+    k = 0
+    for i in range(n):
+        D[i] = a[k]
+        L[i + 1:, i] = a[k + 1:k + n - i]
+        k += n - i
+    return L @ np.diag(D) @ L.T
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 12, 26])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_factor_update_is_rank_one_update(O, n, sign):
+    """mc11a's update of positive definite factors (full rank, no prior solve)
+    yields the factors of H + sig z z^T, checked by dense recomputation."""
+    import ctypes as C
+    L = O.oracle_lib()
+    L.orc_va13_update.restype = None
+    L.orc_va13_update.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_void_p,
+                                  C.POINTER(C.c_int), C.c_int, C.c_double]
+    rng = np.random.default_rng(n)
+    for trial in range(20):
+        a = np.zeros(n * (n + 1) // 2)
+        k = 0
+        for i in range(n):
+            a[k] = rng.uniform(0.5, 2.0)
+            a[k + 1:k + n - i] = rng.normal(scale=0.5, size=n - 1 - i)
+            k += n - i
+        H = _unpack(a, n)
+        z = rng.normal(size=n)
+        # a negative weight small enough to keep the result positive definite
+        lam = np.linalg.eigvalsh(H)[0]
+        sig = rng.uniform(0.1, 2.0) if sign > 0 else -0.5 * lam / (z @ z)
+        want = H + sig * np.outer(z, z)
+        w = np.zeros(n)
+        ir = C.c_int(n)
+        zz = z.copy()
+        L.orc_va13_update(a.ctypes.data, n, zz.ctypes.data, sig, w.ctypes.data, C.byref(ir), 0, 0.0)
+        assert ir.value == n
+        assert np.allclose(_unpack(a, n), want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
