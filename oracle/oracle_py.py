@@ -168,6 +168,17 @@ def ref_lib():
                                          C.POINTER(IGridC), C.POINTER(NLGridC),
                                          C.POINTER(RefTableC), P, C.c_size_t, P, C.c_int,
                                          C.POINTER(C.c_double)]
+        L.ref_pdb.restype = C.c_int
+        L.ref_pdb.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_uint), P, P, P]
+        L.ref_modes.restype = C.c_int
+        L.ref_modes.argtypes = [C.c_char_p, C.c_char_p, C.c_uint, P, P]
+        L.ref_params_file.restype = C.c_int
+        L.ref_params_file.argtypes = [C.c_char_p, C.c_uint, C.POINTER(C.c_uint), P, P, P, P, P]
+        L.ref_dof.restype = C.c_int
+        L.ref_dof.argtypes = [C.c_char_p, C.c_int, C.c_uint, C.c_uint, P, P, C.c_size_t,
+                              C.POINTER(C.c_size_t), P]
+        L.ref_format_results.restype = C.c_size_t
+        L.ref_format_results.argtypes = [P, C.c_size_t, C.c_int, C.c_uint, C.c_uint, P, C.c_size_t]
         L.ref_gridfile_load.restype = C.c_int
         L.ref_gridfile_load.argtypes = [C.c_char_p]
         L.ref_gridfile_info.restype = None
@@ -391,6 +402,61 @@ def ref_score_modes(wl, dofs=None):
                            _p(rec_pot), _p(lig_pot))
     assert rc == 0
     return res, rec_pot, lig_pot
+
+
+def ref_pdb(path, cap=100000):
+    """Reduced PDB through the reference's reader: (xyz SoA, type, charge)."""
+    L = ref_lib()
+    n = C.c_uint(0)
+    xyz = np.zeros(3 * cap, np.float32)
+    t = np.zeros(cap, np.uint32)
+    q = np.zeros(cap, np.float32)
+    if L.ref_pdb(os.fsencode(path), cap, C.byref(n), _p(xyz), _p(t), _p(q)) != 0:
+        raise RuntimeError("capacity")
+    N = n.value
+    return xyz[:3 * N].copy(), t[:N].copy(), q[:N].copy()
+
+
+def ref_modes(pdb, modefile, M, n):
+    """Modes ([x: n*M][y][z]) and force constants through readHMMode."""
+    L = ref_lib()
+    modes = np.zeros(3 * n * M, np.float32)
+    mf = np.zeros(M, np.float32)
+    L.ref_modes(os.fsencode(pdb), os.fsencode(modefile), M, _p(modes), _p(mf))
+    return modes, mf
+
+
+def ref_params_file(path, cap=10000):
+    L = ref_lib()
+    T = C.c_uint(0)
+    arrs = [np.zeros(cap, np.float32), np.zeros(cap, np.float32), np.zeros(cap, np.int32),
+            np.zeros(cap, np.float32), np.zeros(cap, np.float32)]
+    if L.ref_params_file(os.fsencode(path), cap, C.byref(T), *[_p(a) for a in arrs]) != 0:
+        raise RuntimeError("capacity")
+    k = T.value * T.value
+    return T.value, [a[:k].copy() for a in arrs]
+
+
+def ref_dof(path, modes=False, Mr=0, Ml=0, auto_rec=None, auto_lig=None, cap=100000):
+    L = ref_lib()
+    out = np.zeros(cap * (26 if modes else 6), np.float32)
+    n = C.c_size_t(0)
+    ar = np.zeros(3, np.float32) if auto_rec is None else np.asarray(auto_rec, np.float32)
+    al = np.zeros(3, np.float32) if auto_lig is None else np.asarray(auto_lig, np.float32)
+    st = L.ref_dof(os.fsencode(path), 1 if modes else 0, Mr, Ml, _p(ar), _p(al), cap, C.byref(n),
+                   _p(out))
+    if st != 0:
+        raise RuntimeError(f"reference DOF reader status {st}")
+    return out[:n.value * (26 if modes else 6)].reshape(n.value, -1).copy()
+
+
+def ref_format_results(res, modes=False, Mr=0, Ml=0):
+    L = ref_lib()
+    r = np.ascontiguousarray(res, np.float32)
+    ln = L.ref_format_results(_p(r), r.shape[0], 1 if modes else 0, Mr, Ml, None, 0)
+    buf = C.create_string_buffer(max(1, ln))
+    L.ref_format_results(_p(r), r.shape[0], 1 if modes else 0, Mr, Ml, buf, ln)
+    return buf.raw[:ln].decode()
 
 
 def ref_read_gridfile(path):

@@ -15,6 +15,9 @@
 //                   with n_rec != n_lig)
 //  param table    : fileIO/readFile.cpp:439-527 on a written attract.par
 //  plateau ratios : model/NLGrid.tpp:44-51
+//  file formats   : reduced PDB, mode files, attract.par, DOF files through
+//                   readFile.cpp's readers, DOFConverter and
+//                   transformDOF_glob2rec, and the result operator<<
 //  grid files     : fileIO/readFile.cpp:260-422 (createGridFromGridFile over
 //                   attract::Grid::read_std) on a file written by this
 //                   project's att_gridfile_write
@@ -33,7 +36,11 @@
 #include <unistd.h>
 #include <vector>
 
+#include <sstream>
+
 #include "CPUEnergyService6D.h"
+#include "DOFConverter.h"
+#include "DOFTransform.h"
 #include "DataManager.h"
 #include "GridUnion.h"
 #include "IntrplGrid.h"
@@ -464,5 +471,121 @@ void ref_gridfile_copy(float* inner, float* outer, uint32_t* desc, uint32_t* lis
 }
 
 void ref_gridfile_free() { g_file_grid.reset(); }
+
+// Reduced PDB through createProteinFromPDB<float>.
+int ref_pdb(const char* path, unsigned cap, unsigned* n, float* xyz, unsigned* type, float* charge) {
+    auto p = createProteinFromPDB<float>(std::string(path));
+    *n = p->numAtoms();
+    if (*n > cap) return -1;
+    const unsigned N = *n;
+    std::copy(p->xPos(), p->xPos() + N, xyz);
+    std::copy(p->yPos(), p->yPos() + N, xyz + N);
+    std::copy(p->zPos(), p->zPos() + N, xyz + 2 * N);
+    std::copy(p->type(), p->type() + N, type);
+    std::copy(p->charge(), p->charge() + N, charge);
+    return 0;
+}
+
+// Normal modes through readHMMode<float> on the body of `pdb`.
+int ref_modes(const char* pdb, const char* modefile, unsigned M, float* modes, float* modeForce) {
+    auto p = createProteinFromPDB<float>(std::string(pdb));
+    p->setNumModes(M);
+    readHMMode<float>(p, std::string(modefile));
+    const size_t nm = (size_t)p->numAtoms() * M;
+    std::copy(p->xModes(), p->xModes() + nm, modes);
+    std::copy(p->yModes(), p->yModes() + nm, modes + nm);
+    std::copy(p->zModes(), p->zModes() + nm, modes + 2 * nm);
+    std::copy(p->modeForce(), p->modeForce() + M, modeForce);
+    return 0;
+}
+
+// attract.par through createParamTableFromFile<float>: T*T x {rc, ac, ipon, rmin2, emin}.
+int ref_params_file(const char* path, unsigned cap, unsigned* T, float* rc, float* ac, int* ipon,
+                    float* rmin2, float* emin) {
+    auto tab = createParamTableFromFile<float>(std::string(path));
+    *T = tab->numTypes();
+    if ((size_t)*T * *T > cap) return -1;
+    for (unsigned k = 0; k < *T * *T; ++k) {
+        const auto& q = tab->table()[k];
+        rc[k] = q.rc; ac[k] = q.ac; ipon[k] = q.ipon; rmin2[k] = q.rmin2; emin[k] = q.emin;
+    }
+    return 0;
+}
+
+// DOF file -> ligand DOFs in the receptor frame, as Configurator_6D[_Modes]
+// does (service/Configurator_6D.tpp:56-100): header, readDOF, DOFConverter,
+// pivots (explicit, or the given auto pivots), transformDOF_glob2rec.
+int ref_dof(const char* path, int modes, unsigned Mr, unsigned Ml, const float* autoRec,
+            const float* autoLig, size_t cap, size_t* n, float* out) {
+    DOFHeader<float> h = readDOFHeader<float>(std::string(path));
+    auto raw = readDOF(std::string(path));
+    if (raw.size() != 2) return -1;
+    if (h.auto_pivot) {
+        h.pivots.push_back(Vec3<float>(autoRec[0], autoRec[1], autoRec[2]));
+        h.pivots.push_back(Vec3<float>(autoLig[0], autoLig[1], autoLig[2]));
+    }
+    if (h.pivots.size() != 2) return -2;
+    if (!modes) {
+        auto m = DOFConverter<float>(raw);
+        transformDOF_glob2rec(m[0], m[1], h.pivots[0], h.pivots[1], h.centered_receptor, h.centered_ligands);
+        *n = m[1].size();
+        if (*n > cap) return -3;
+        for (size_t i = 0; i < *n; ++i) {
+            float* d = out + 6 * i;
+            d[0] = m[1][i].pos.x; d[1] = m[1][i].pos.y; d[2] = m[1][i].pos.z;
+            d[3] = m[1][i].ang.x; d[4] = m[1][i].ang.y; d[5] = m[1][i].ang.z;
+        }
+    } else {
+        Common_Modes::numModesRec = Mr;
+        Common_Modes::numModesLig = Ml;
+        auto m = DOFConverter_Modes<float>(raw);
+        transformDOF_glob2rec(m[0], m[1], h.pivots[0], h.pivots[1], h.centered_receptor, h.centered_ligands);
+        *n = m[1].size();
+        if (*n > cap) return -3;
+        for (size_t i = 0; i < *n; ++i) {
+            float* d = out + 26 * i;
+            const auto& q = m[1][i];
+            d[0] = q._6D.pos.x; d[1] = q._6D.pos.y; d[2] = q._6D.pos.z;
+            d[3] = q._6D.ang.x; d[4] = q._6D.ang.y; d[5] = q._6D.ang.z;
+            for (unsigned k = 0; k < 10; ++k) {
+                d[6 + k] = k < Mr ? q.modesRec[k] : 0.f;
+                d[16 + k] = k < Ml ? q.modesLig[k] : 0.f;
+            }
+        }
+    }
+    return 0;
+}
+
+// The apps' result lines: `std::cout << res << std::endl` per pose.
+size_t ref_format_results(const float* res, size_t n, int modes, unsigned Mr, unsigned Ml, char* buf,
+                          size_t cap) {
+    std::ostringstream s;
+    for (size_t i = 0; i < n; ++i) {
+        if (!modes) {
+            const float* e = res + 7 * i;
+            Result_6D<float> r;
+            r.E = e[0];
+            r.pos = Vec3<float>(e[1], e[2], e[3]);
+            r.ang = Vec3<float>(e[4], e[5], e[6]);
+            s << r << std::endl;
+        } else {
+            Common_Modes::numModesRec = Mr;
+            Common_Modes::numModesLig = Ml;
+            const float* e = res + 27 * i;
+            Result_6D_Modes<float> r;
+            r._6D.E = e[0];
+            r._6D.pos = Vec3<float>(e[1], e[2], e[3]);
+            r._6D.ang = Vec3<float>(e[4], e[5], e[6]);
+            for (int k = 0; k < 10; ++k) {
+                r.modesRec[k] = e[7 + k];
+                r.modesLig[k] = e[17 + k];
+            }
+            s << r << std::endl;
+        }
+    }
+    const std::string str = s.str();
+    if (buf) std::memcpy(buf, str.data(), std::min(cap, str.size()));
+    return str.size();
+}
 
 }  // extern "C"

@@ -21,6 +21,7 @@ def declared(header, prefix):
     ("attract_gen.h", "libattract_gen.so", "attg_"),
     ("attract_serve.h", "libattract_service.so", "att_server_"),
     ("attract_io.h", "libattract_service.so", "att_gridfile_"),
+    ("attract_formats.h", "libattract_service.so", "att_"),
 ])
 def test_exports(header, lib, prefix):
     names = declared(header, prefix)
