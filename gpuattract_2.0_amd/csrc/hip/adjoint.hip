@@ -536,18 +536,19 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     if (s_pg)
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
     const unsigned n = P.nDev ? min(P.nDev[0], P.n) : P.n;
-    // With a spatial order, workgroup b (on XCD b % 8) walks the (b % 8)-th
-    // eighth of it (the launch then has at least 8 workgroups); otherwise the
-    // poses in input order, strided by the grid.
+    // With a spatial order (and at least 8 workgroups), workgroup b (on XCD
+    // b % 8) walks the (b % 8)-th eighth of it; otherwise the poses in input
+    // order, strided by the grid.
     unsigned first = blockIdx.x, end = n, step = gridDim.x;
-    if (P.order) {
+    const bool ordered = P.order && gridDim.x >= 8;
+    if (ordered) {
         const unsigned x = blockIdx.x & 7u;
         step = (gridDim.x - x + 7u) >> 3;
         first = (unsigned)(((unsigned long long)n * x) >> 3) + (blockIdx.x >> 3);
         end = (unsigned)(((unsigned long long)n * (x + 1)) >> 3);
     }
     for (unsigned i = first; i < end; i += step)
-        adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, dyn, tab, s_pg);
+        adjoint_pose<MM, WAVES>(P, ordered ? P.order[i] : i, S, dyn, tab, s_pg);
     if (s_pg) {
         __syncthreads();
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x)
