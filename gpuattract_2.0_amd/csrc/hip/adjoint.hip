@@ -557,37 +557,44 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
 }
 
 // Batched adjoint over many pairs: persistent workgroups take tiles (pair,
-// first pose, count) from a shared counter; the pair's descriptor is read
-// with scalar loads and its table staged in LDS when the pair changes.
+// first pose, count) from eight queues, one per XCD: workgroup b (on XCD b % 8)
+// drains queue b % 8, then helps with the others.  Queue x holds the x-th
+// eighth of every pair's poses in that pair's spatial order, so one XCD's L2
+// sees neighbouring poses.  The pair's descriptor is read with scalar loads
+// and its table staged in LDS when the pair changes.
 template <int MM, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint_batch(
-    const PairDesc* __restrict__ pairs, const BatchTile* __restrict__ tiles, unsigned ntiles,
-    unsigned* __restrict__ counter) {
+    const PairDesc* __restrict__ pairs, const BatchTile* __restrict__ tiles,
+    const unsigned* __restrict__ queueStart, unsigned* __restrict__ counters) {
     extern __shared__ att_pair_params s_dyn[];
     __shared__ PairLDS<WAVES> S;
     __shared__ unsigned s_tile;
     char* dyn = reinterpret_cast<char*>(s_dyn);
     unsigned cur = 0xFFFFFFFFu;
     const att_pair_params* tab = nullptr;
-    for (;;) {
-        __syncthreads();  // s_tile and the table of the previous tile are no longer read
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
-        __syncthreads();
-        const unsigned t = __builtin_amdgcn_readfirstlane(s_tile);
-        if (t >= ntiles) break;
-        const BatchTile tile = tiles[t];
-        const PairDesc& P = pairs[tile.pair];
-        if (tile.pair != cur) {
-            tab = stage_table(P.par, s_dyn);  // made visible by adjoint_pose's first barrier
-            cur = tile.pair;
+    for (unsigned q = 0; q < kBatchQueues; ++q) {
+        const unsigned x = (blockIdx.x + q) % kBatchQueues;
+        const unsigned lo = queueStart[x], hi = queueStart[x + 1];
+        for (;;) {
+            __syncthreads();  // s_tile and the table of the previous tile are no longer read
+            if (threadIdx.x == 0) s_tile = lo + atomicAdd(&counters[x], 1u);
+            __syncthreads();
+            const unsigned t = __builtin_amdgcn_readfirstlane(s_tile);
+            if (t >= hi) break;
+            const BatchTile tile = tiles[t];
+            const PairDesc& P = pairs[tile.pair];
+            if (tile.pair != cur) {
+                tab = stage_table(P.par, s_dyn);  // made visible by adjoint_pose's first barrier
+                cur = tile.pair;
+            }
+            for (unsigned i = tile.first; i < tile.first + tile.count; ++i)
+                adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, dyn, tab, nullptr);
         }
-        for (unsigned pose = tile.first; pose < tile.first + tile.count; ++pose)
-            adjoint_pose<MM, WAVES>(P, pose, S, dyn, tab, nullptr);
     }
 }
 
-__global__ void k_reset_counter(unsigned* counter) {
-    if (threadIdx.x == 0) counter[0] = 0u;
+__global__ void k_reset_counters(unsigned* counters, unsigned n) {
+    if (threadIdx.x < n) counters[threadIdx.x] = 0u;
 }
 
 __global__ void k_fill_zero(float* p, unsigned n) {
@@ -690,12 +697,12 @@ hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsig
 }
 
 hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,
-                                unsigned ntiles, unsigned* counter, unsigned blocks, size_t lds,
-                                hipStream_t s) {
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, counter);
+                                const unsigned* queueStart, unsigned ntiles, unsigned* counters,
+                                unsigned blocks, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, s, counters, kBatchQueues);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || ntiles == 0) return e;
-    void* args[] = {&pairs, &tiles, &ntiles, &counter};
+    void* args[] = {&pairs, &tiles, &queueStart, &counters};
     return hipLaunchKernel(adjoint_kernel_for(mm, waves, true), dim3(blocks), dim3(64 * waves), args,
                            lds, s);
 }

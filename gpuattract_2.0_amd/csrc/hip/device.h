@@ -487,10 +487,14 @@ __host__ __device__ inline size_t adjoint_pg_offset(unsigned tabSlots, unsigned 
     return adjoint_queue_offset(tabSlots, ligN) + 2 * (((size_t)ligN + 1) & ~(size_t)1);
 }
 
-// One unit of batched work: `count` consecutive poses of pair `pair`.
+// One unit of batched work: `count` consecutive poses of pair `pair` (in the
+// pair's spatial order when it has one).
 struct BatchTile {
     unsigned pair, first, count;
 };
+
+// Tile queues of a batched launch: one per XCD.
+constexpr unsigned kBatchQueues = 8;
 
 struct NoReaction {
     __device__ void operator()(unsigned, float, float, float) const {}

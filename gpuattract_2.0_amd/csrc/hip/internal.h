@@ -47,11 +47,12 @@ unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds);
 // One pair: persistent grid of `blocks` workgroups of 64*waves threads.
 hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsigned blocks,
                           int waves, size_t lds, hipStream_t s);
-// Many pairs of one mode-capacity class: resets the tile counter, then the
-// persistent batch kernel (graph-capturable: no memset, no host sync).
+// Many pairs of one mode-capacity class: resets the kBatchQueues tile
+// counters, then the persistent batch kernel over the tile queues
+// [queueStart[x], queueStart[x+1]) (graph-capturable: no memset, no host sync).
 hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,
-                                unsigned ntiles, unsigned* counter, unsigned blocks, size_t lds,
-                                hipStream_t s);
+                                const unsigned* queueStart, unsigned ntiles, unsigned* counters,
+                                unsigned blocks, size_t lds, hipStream_t s);
 
 // One device-resident minimization run (minimize.hip): per-pose VA13 states,
 // the next round's request list, count and points, and the results.

@@ -946,6 +946,10 @@ att_status att_event_sync(att_ctx* c, void* ev) {
 // k_adjoint_batch launch whose shape (waves per pose, poses per tile) the
 // autotuner picks by timing the candidates on the bucket's own work.  The
 // launches are captured into one hipGraph, replayed by att_batch_run.
+// Each pair's poses are put in spatial order when the batch is created (from
+// the DOFs its buffers hold then; the order changes no result, only locality),
+// and each bucket's tiles are dealt into one queue per XCD: queue x holds the
+// x-th eighth of every pair's poses.
 struct att_batch {
     struct Bucket {
         unsigned mm = 0;
@@ -954,8 +958,9 @@ struct att_batch {
         size_t lds = 0;               // dynamic LDS: the largest pair of the bucket
         int waves = 4;
         unsigned tile = 16;
-        std::vector<BatchTile> tiles;
-        BatchTile* d_tiles = nullptr;
+        std::vector<BatchTile> tiles;  // queue by queue
+        unsigned queueStart[kBatchQueues + 1] = {};
+        BatchTile* d_tiles = nullptr;  // the tiles, then queueStart
         size_t d_tiles_cap = 0;
         float ms = -1.f;              // time of the chosen shape when tuned
     };
@@ -963,7 +968,8 @@ struct att_batch {
     std::vector<double> cost;         // per pair, relative cost of one pose
     PairDesc* d_pairs = nullptr;
     std::vector<Bucket> buckets;
-    unsigned* d_counters = nullptr;
+    unsigned* d_counters = nullptr;   // kBatchQueues per bucket
+    unsigned* d_order = nullptr;      // spatial order of the pairs that have one
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -980,16 +986,24 @@ att_status batch_tiles(att_ctx* c, att_batch* b, att_batch::Bucket& k) {
     // expensive pairs first, so the last tiles taken are the cheap ones
     std::stable_sort(order.begin(), order.end(),
                      [&](unsigned x, unsigned y) { return b->cost[x] > b->cost[y]; });
-    for (unsigned p : order) {
-        const unsigned n = b->pairs[p].n;
-        for (unsigned f = 0; f < n; f += k.tile)
-            k.tiles.push_back(BatchTile{p, f, std::min(k.tile, n - f)});
+    for (unsigned x = 0; x < kBatchQueues; ++x) {
+        k.queueStart[x] = (unsigned)k.tiles.size();
+        for (unsigned p : order) {
+            const unsigned n = b->pairs[p].n;
+            const unsigned lo = (unsigned)((uint64_t)n * x / kBatchQueues);
+            const unsigned hi = (unsigned)((uint64_t)n * (x + 1) / kBatchQueues);
+            for (unsigned f = lo; f < hi; f += k.tile)
+                k.tiles.push_back(BatchTile{p, f, std::min(k.tile, hi - f)});
+        }
     }
-    const size_t bytes = sizeof(BatchTile) * std::max<size_t>(k.tiles.size(), 1);
+    k.queueStart[kBatchQueues] = (unsigned)k.tiles.size();
+    const size_t tileBytes = sizeof(BatchTile) * k.tiles.size();
+    const size_t bytes = tileBytes + sizeof(k.queueStart);
     att_status st;
     if ((st = grow(c, reinterpret_cast<void**>(&k.d_tiles), &k.d_tiles_cap, bytes)) != ATT_OK) return st;
-    ATT_HIP(c, hipMemcpy(k.d_tiles, k.tiles.data(), sizeof(BatchTile) * k.tiles.size(),
-                         hipMemcpyHostToDevice));
+    if (tileBytes) ATT_HIP(c, hipMemcpy(k.d_tiles, k.tiles.data(), tileBytes, hipMemcpyHostToDevice));
+    ATT_HIP(c, hipMemcpy(reinterpret_cast<char*>(k.d_tiles) + tileBytes, k.queueStart,
+                         sizeof(k.queueStart), hipMemcpyHostToDevice));
     b->dirty = true;
     return ATT_OK;
 }
@@ -999,8 +1013,41 @@ hipError_t batch_launch(att_ctx* c, att_batch* b, size_t i, hipStream_t s) {
     const size_t resident =
         (size_t)c->numCU * adjoint_blocks_per_cu(k.mm, k.waves, true, k.lds);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(k.tiles.size(), resident));
-    return launch_adjoint_batch(k.mm, k.waves, b->d_pairs, k.d_tiles, (unsigned)k.tiles.size(),
-                                b->d_counters + i, blocks, k.lds, s);
+    const unsigned* qs = reinterpret_cast<const unsigned*>(k.d_tiles + k.tiles.size());
+    return launch_adjoint_batch(k.mm, k.waves, b->d_pairs, k.d_tiles, qs, (unsigned)k.tiles.size(),
+                                b->d_counters + kBatchQueues * i, blocks, k.lds, s);
+}
+
+// Spatial order of every pair with enough poses to span several tiles per
+// queue, sorted from the DOFs the pair's buffer holds now (in a scratch buffer
+// of its own, freed before returning); sets PairDesc::order before the
+// descriptors go to the device.
+constexpr unsigned kBatchOrderMin = 256;
+
+att_status batch_order(att_ctx* c, att_batch* b) {
+    if (!c->poseOrder) return ATT_OK;
+    size_t total = 0, scratch = 0;
+    for (const PairDesc& P : b->pairs)
+        if (P.n >= kBatchOrderMin) {
+            total += P.n;
+            scratch = std::max(scratch, pose_order_scratch(P.n));
+        }
+    if (total == 0) return ATT_OK;
+    ATT_HIP(c, hipMalloc(&b->d_order, sizeof(unsigned) * total));
+    void* tmp = nullptr;
+    ATT_HIP(c, hipMalloc(&tmp, scratch));
+    hipError_t e = hipSuccess;
+    unsigned* o = b->d_order;
+    for (PairDesc& P : b->pairs)
+        if (P.n >= kBatchOrderMin && e == hipSuccess) {
+            e = launch_pose_order(P.dofs, P.dofStride, P.n, tmp, scratch, o, c->stream);
+            P.order = o;
+            o += P.n;
+        }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(c, ATT_ERR_HIP, std::string("pose order: ") + hipGetErrorString(e));
+    return ATT_OK;
 }
 
 void batch_free_graph(att_batch* b) {
@@ -1014,11 +1061,13 @@ void batch_free_graph(att_batch* b) {
 
 extern "C" {
 
-att_status att_batch_create(att_ctx* c, const att_batch_pair* in, unsigned npairs, att_batch** out) {
-    if (!c) return ATT_ERR_INVALID;
-    if (!out || (npairs && !in)) return fail(c, ATT_ERR_INVALID, "null argument");
-    ATT_HIP(c, hipSetDevice(c->device));
-    std::unique_ptr<att_batch> b(new att_batch);
+att_status att_batch_destroy(att_ctx* c, att_batch* b);
+
+}  // extern "C"
+
+namespace {
+
+att_status batch_fill(att_ctx* c, att_batch* b, const att_batch_pair* in, unsigned npairs) {
     std::map<std::pair<unsigned, int>, size_t> index;
     for (unsigned i = 0; i < npairs; ++i) {
         const att_batch_pair& q = in[i];
@@ -1060,15 +1109,34 @@ att_status att_batch_create(att_ctx* c, const att_batch_pair* in, unsigned npair
         b->cost.push_back((double)lig->d.n * (1.0 + 0.25 * m) + 0.05 * rec->d.n);
     }
     att_status st;
+    if ((st = batch_order(c, b)) != ATT_OK) return st;
     if (npairs) {
         ATT_HIP(c, hipMalloc(&b->d_pairs, sizeof(PairDesc) * npairs));
         ATT_HIP(c, hipMemcpy(b->d_pairs, b->pairs.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice));
     }
-    ATT_HIP(c, hipMalloc(&b->d_counters, sizeof(unsigned) * std::max<size_t>(b->buckets.size(), 1)));
+    ATT_HIP(c, hipMalloc(&b->d_counters,
+                         sizeof(unsigned) * kBatchQueues * std::max<size_t>(b->buckets.size(), 1)));
     for (auto& k : b->buckets)
-        if ((st = batch_tiles(c, b.get(), k)) != ATT_OK) return st;
+        if ((st = batch_tiles(c, b, k)) != ATT_OK) return st;
     ATT_HIP(c, hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking));
-    *out = b.release();
+    return ATT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+att_status att_batch_create(att_ctx* c, const att_batch_pair* in, unsigned npairs, att_batch** out) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!out || (npairs && !in)) return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    att_batch* b = new att_batch;
+    const att_status st = batch_fill(c, b, in, npairs);
+    if (st != ATT_OK) {
+        (void)att_batch_destroy(c, b);  // frees what was allocated before the failure
+        return st;
+    }
+    *out = b;
     return ATT_OK;
 }
 
@@ -1080,6 +1148,7 @@ att_status att_batch_destroy(att_ctx* c, att_batch* b) {
         if (k.d_tiles) (void)hipFree(k.d_tiles);
     if (b->d_pairs) (void)hipFree(b->d_pairs);
     if (b->d_counters) (void)hipFree(b->d_counters);
+    if (b->d_order) (void)hipFree(b->d_order);
     if (b->capture) (void)hipStreamDestroy(b->capture);
     delete b;
     return ATT_OK;

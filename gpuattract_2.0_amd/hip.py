@@ -412,6 +412,13 @@ class Context:
         self._chk(self.L.att_memcpy_h2d(self.h, buf.ptr, _p(a), a.nbytes, None))
         return buf
 
+    def write(self, buf, arr):
+        """Copy a host array into the start of an existing device buffer."""
+        a = np.ascontiguousarray(arr)
+        if a.nbytes > buf.nbytes:
+            raise ValueError("array larger than the device buffer")
+        self._chk(self.L.att_memcpy_h2d(self.h, buf.ptr, _p(a), a.nbytes, None))
+
     def to_host(self, buf, shape, dtype=np.float32):
         out = np.empty(shape, dtype)
         self._chk(self.L.att_memcpy_d2h(self.h, _p(out), buf.ptr, out.nbytes, None))

@@ -134,3 +134,26 @@ def test_batch_errors(ctx, staged):
     empty.run()
     ctx.sync()
     empty.close()
+
+
+def test_spatial_order_changes_no_result(ctx, staged):
+    """Pairs of 256 poses or more are scored in their spatial order, dealt into
+    per-XCD queues; every pose still lands in its own slot, so results with the
+    order switched off are the same bits."""
+    assert any(s["n"] >= 256 for s in staged)
+    runs = []
+    for enabled in (False, True):
+        ctx.set_pose_order(enabled)
+        b = _batch(ctx, staged)
+        for k in range(len(b.buckets())):
+            b.set_shape(k, 2, 4)
+        for s in staged:  # no slot keeps the previous run's value
+            ctx.write(s["d_out"], np.full((s["n"], s["ow"]), np.nan, np.float32))
+        ctx.sync()
+        b.run()
+        ctx.sync()
+        runs.append(_results(ctx, staged))
+        b.close()
+    ctx.set_pose_order(True)
+    for a, c, s in zip(runs[0], runs[1], staged):
+        assert np.array_equal(a, c), s["wl"].name
