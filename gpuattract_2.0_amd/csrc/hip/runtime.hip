@@ -942,7 +942,7 @@ att_status att_event_sync(att_ctx* c, void* ev) {
 // ---------------------------------------------------------------- batches
 // Many receptor-ligand pairs of uneven sizes scored by a few persistent
 // launches.  Pairs are grouped into buckets by mode capacity (0, <=5, <=10)
-// and ligand size (<192, <640, larger beads); each bucket runs one
+// and ligand size (<192, <384, <640, <1024, larger beads); each bucket runs one
 // k_adjoint_batch launch whose shape (waves per pose, poses per tile) the
 // autotuner picks by timing the candidates on the bucket's own work.  The
 // launches are captured into one hipGraph, replayed by att_batch_run.
@@ -978,7 +978,12 @@ struct att_batch {
 
 namespace {
 
-int size_class(unsigned ligN) { return ligN < 192 ? 0 : ligN < 640 ? 1 : 2; }
+// Ligand size classes: the bucket's LDS is its largest ligand's, so classes
+// narrow enough that a pair runs near its own occupancy (timed: five classes
+// beat three by 2.7% on 64 uneven pairs).
+int size_class(unsigned ligN) {
+    return ligN < 192 ? 0 : ligN < 384 ? 1 : ligN < 640 ? 2 : ligN < 1024 ? 3 : 4;
+}
 
 att_status batch_tiles(att_ctx* c, att_batch* b, att_batch::Bucket& k) {
     k.tiles.clear();
@@ -1244,6 +1249,8 @@ att_status att_batch_run(att_ctx* c, att_batch* b, void* stream) {
     if (b->dirty) {
         batch_free_graph(b);
         ATT_HIP(c, hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal));
+        // one bucket after the other: buckets on parallel graph branches mix
+        // pairs in the caches (timed 1.6% slower on two branches, 6% on four)
         hipError_t e = hipSuccess;
         for (size_t i = 0; i < b->buckets.size() && e == hipSuccess; ++i) e = batch_launch(c, b, i, b->capture);
         hipGraph_t g = nullptr;

@@ -279,7 +279,7 @@ typedef struct {
 } att_batch_pair;
 typedef struct {
     unsigned modeCapacity;   /* 0, 5 or 10 */
-    int sizeClass;           /* ligand beads: 0 (<192), 1 (<640), 2 */
+    int sizeClass;           /* ligand beads: 0 (<192), 1 (<384), 2 (<640), 3 (<1024), 4 */
     unsigned numPairs, numTiles;
     int wavesPerPose;        /* 1, 2 or 4 */
     unsigned posesPerTile;
