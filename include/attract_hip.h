@@ -267,7 +267,11 @@ att_status att_event_sync(att_ctx* ctx, void* ev);
  * pose, poses per work tile) is picked by att_batch_tune, and the launches are
  * captured into one hipGraph that att_batch_run replays.  Per pose the output
  * is the adjoint's (att_score_adjoint_dev without parameter gradients).  The
- * device buffers named by the pairs must stay valid while the batch exists. */
+ * device buffers named by the pairs must stay valid while the batch exists.
+ * att_batch_create sorts the poses of every pair of 256 or more poses into a
+ * spatial order from the DOFs its buffer holds at that moment (synchronously);
+ * later runs keep that order whatever the buffer then holds, which changes
+ * only their speed, never a result. */
 typedef struct att_batch att_batch;
 typedef struct {
     att_handle rec, lig, grid, params;  /* uploaded receptor, ligand, receptor grid, table */
