@@ -124,18 +124,32 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
 }
 
 // Grid term of one ligand bead: inner grid, else outer grid, else nothing
-// (model/interpolation.h:56-102).
+// (model/interpolation.h:56-102).  Each lane selects its grid's geometry and
+// runs the one lookup path, so a wave whose beads straddle the inner grid's
+// boundary does not walk the lookup (and the basis loop) twice.
 __device__ __forceinline__ float grid_adj(const DGrid& inner, const DGrid& outer,
                                           const DBasis& bin, const DBasis& bout, const DPar& par,
                                           float px, float py, float pz, unsigned mtype,
                                           float charge, float3& G, const PGrad& pg) {
     G = make_float3(0.f, 0.f, 0.f);
     if (mtype == 0) return 0.f;
-    if (!grid_out(inner, px, py, pz))
-        return grid_adj_one(inner, bin, px, py, pz, mtype, charge, G, pg);
-    if (!grid_out(outer, px, py, pz))
-        return grid_adj_one(outer, bout, px, py, pz, mtype, charge, G, pg);
-    return 0.f;
+    const bool in = !grid_out(inner, px, py, pz);
+    if (!in && grid_out(outer, px, py, pz)) return 0.f;
+    DGrid g{};
+    g.oct = in ? inner.oct : outer.oct;
+    g.nx = in ? inner.nx : outer.nx;
+    g.ny = in ? inner.ny : outer.ny;
+    g.nz = in ? inner.nz : outer.nz;
+    g.lox = in ? inner.lox : outer.lox;
+    g.loy = in ? inner.loy : outer.loy;
+    g.loz = in ? inner.loz : outer.loz;
+    g.dVox = in ? inner.dVox : outer.dVox;
+    g.dVox_inv = in ? inner.dVox_inv : outer.dVox_inv;
+    g.vol_inv = in ? inner.vol_inv : outer.vol_inv;
+    DBasis b{};
+    b.oct = in ? bin.oct : bout.oct;
+    b.nb = in ? bin.nb : bout.nb;
+    return grid_adj_one(g, b, px, py, pz, mtype, charge, G, pg);
 }
 
 // LJ derivatives with respect to (rc, ac, emin) at squared distance dr2.

@@ -487,6 +487,15 @@ __host__ __device__ inline size_t adjoint_pg_offset(unsigned tabSlots, unsigned 
     return adjoint_queue_offset(tabSlots, ligN) + 2 * (((size_t)ligN + 1) & ~(size_t)1);
 }
 
+// Slot of workgroup b among nb in a non-persistent launch such that the
+// workgroups of one XCD (dispatched round-robin: b % 8) take one contiguous
+// range of slots: XCD x holds ceil((nb - x) / 8) workgroups, and its range
+// starts after those of XCDs 0..x-1.  A bijection of [0, nb).
+__device__ __forceinline__ unsigned xcd_slot(unsigned b, unsigned nb) {
+    const unsigned x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // One unit of batched work: `count` consecutive poses of pair `pair` (in the
 // pair's spatial order when it has one).
 struct BatchTile {

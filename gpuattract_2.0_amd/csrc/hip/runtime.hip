@@ -83,6 +83,9 @@ struct att_ctx {
     void* d_ord = nullptr;  // spatial pose order: permutation + sort scratch
     size_t d_ord_cap = 0;
     bool poseOrder = true;
+    // Recorded after the last kernel that used d_pg or d_ord: a call on
+    // another stream waits for it before reusing them.
+    hipEvent_t scratchEv = nullptr;
 };
 
 namespace {
@@ -286,6 +289,10 @@ att_status att_ctx_create(int hip_device, att_ctx** out) {
     auto c = std::make_unique<att_ctx>();
     c->device = hip_device;
     if (hipStreamCreate(&c->stream) != hipSuccess) return ATT_ERR_HIP;
+    if (hipEventCreateWithFlags(&c->scratchEv, hipEventDisableTiming) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        return ATT_ERR_HIP;
+    }
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, hip_device) == hipSuccess && lds > 0)
         c->maxLdsPerBlock = (size_t)lds;
@@ -307,6 +314,7 @@ void att_ctx_destroy(att_ctx* c) {
     if (c->d_ord) (void)hipFree(c->d_ord);
     if (c->d_w) (void)hipFree(c->d_w);
     if (c->d_g) (void)hipFree(c->d_g);
+    if (c->scratchEv) (void)hipEventDestroy(c->scratchEv);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -458,6 +466,7 @@ static att_status resolve(att_ctx* c, att_handle hr, att_handle hl, att_handle h
 
 static att_status grow(att_ctx* c, void** p, size_t* cap, size_t bytes) {
     if (bytes <= *cap) return ATT_OK;
+    if (*p) ATT_HIP(c, hipEventSynchronize(c->scratchEv));  // its last user has finished
     if (*p) ATT_HIP(c, hipFree(*p));
     *p = nullptr;
     *cap = 0;
@@ -465,6 +474,27 @@ static att_status grow(att_ctx* c, void** p, size_t* cap, size_t bytes) {
     *cap = bytes;
     return ATT_OK;
 }
+
+// Spatial order of n poses (DOF rows of `stride` floats) into the context's
+// scratch on stream s, the stream having waited for the scratch's previous
+// user; *order stays null when the order is off or n is below minPoses.  The
+// caller records scratchEv on s after its last kernel that reads the order.
+static att_status pose_order(att_ctx* c, const float* d_dofs, unsigned stride, size_t n,
+                             size_t minPoses, hipStream_t s, const unsigned** order) {
+    *order = nullptr;
+    if (!c->poseOrder || n < minPoses || n == 0) return ATT_OK;
+    const size_t scratch = pose_order_scratch((unsigned)n);
+    att_status st;
+    if ((st = grow(c, &c->d_ord, &c->d_ord_cap, scratch + sizeof(unsigned) * n)) != ATT_OK) return st;
+    unsigned* o = reinterpret_cast<unsigned*>(static_cast<char*>(c->d_ord) + scratch);
+    ATT_HIP(c, hipStreamWaitEvent(s, c->scratchEv, 0));
+    ATT_HIP(c, launch_pose_order(d_dofs, stride, (unsigned)n, c->d_ord, scratch, o, s));
+    *order = o;
+    return ATT_OK;
+}
+
+// Forward kernels sort their poses spatially from this many poses on.
+constexpr size_t kForwardOrderMin = 4096;
 
 static att_status ensure_staging(att_ctx* c, size_t inBytes, size_t outBytes) {
     if (inBytes > c->d_in_cap) {
@@ -499,9 +529,13 @@ att_status att_score_rigid_dev(att_ctx* c, att_handle hr, att_handle hl, att_han
     if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
     if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const unsigned* order = nullptr;
+    if ((st = pose_order(c, reinterpret_cast<const float*>(d_dofs), 6, n, kForwardOrderMin, s, &order)) != ATT_OK)
+        return st;
     ATT_HIP(c, launch_rigid(rec->d, lig->d, g->inner, g->outer, g->nl, p->d,
                             reinterpret_cast<const float*>(d_dofs), reinterpret_cast<float*>(d_out),
-                            (unsigned)n, s));
+                            (unsigned)n, order, s));
+    if (order) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
 
@@ -553,9 +587,13 @@ att_status att_score_modes_dev(att_ctx* c, att_handle hr, att_handle hl, att_han
     if (!gl) return fail(c, ATT_ERR_HANDLE, "unknown ligand grid handle");
     if ((st = check_modes(c, rec, lig, gr, gl, p)) != ATT_OK) return st;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const unsigned* order = nullptr;
+    if ((st = pose_order(c, reinterpret_cast<const float*>(d_dofs), 26, n, kForwardOrderMin, s, &order)) != ATT_OK)
+        return st;
     ATT_HIP(c, launch_modes(rec->d, lig->d, gr->inner, gr->outer, gr->nl, gl->inner, gl->outer,
                             p->d, reinterpret_cast<const float*>(d_dofs),
-                            reinterpret_cast<float*>(d_out), (unsigned)n, s));
+                            reinterpret_cast<float*>(d_out), (unsigned)n, order, s));
+    if (order) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
 
@@ -653,17 +691,15 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
     P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
     P.n = (unsigned)n; P.tabSlots = slots;
-    if (c->poseOrder && blocks >= 8 && n >= 4 * (size_t)blocks) {
-        const size_t scratch = pose_order_scratch((unsigned)n);
-        if ((st = grow(c, &c->d_ord, &c->d_ord_cap, scratch + sizeof(unsigned) * n)) != ATT_OK) return st;
-        unsigned* order = reinterpret_cast<unsigned*>(static_cast<char*>(c->d_ord) + scratch);
-        ATT_HIP(c, launch_pose_order(d_dofs, P.dofStride, (unsigned)n, c->d_ord, scratch, order, s));
-        P.order = order;
-    }
+    if (blocks >= 8 && (st = pose_order(c, d_dofs, P.dofStride, n, 4 * (size_t)blocks, s, &P.order)) != ATT_OK)
+        return st;
+    const bool scratch = pgrad || P.order;
+    if (pgrad && !P.order) ATT_HIP(c, hipStreamWaitEvent(s, c->scratchEv, 0));
     ATT_HIP(c, launch_adjoint(P, modes != 0, pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks,
                               kWaves, dyn, s));
     if (pgrad)
         ATT_HIP(c, launch_pgrad_reduce(static_cast<const float*>(c->d_pg), blocks, cols, d_pgrad, s));
+    if (scratch) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
 

@@ -24,13 +24,16 @@ namespace att {
 
 __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer,
                                                DNL nl, DPar par, const float* __restrict__ dofs,
-                                               float* __restrict__ res, unsigned n) {
+                                               float* __restrict__ res, unsigned n,
+                                               const unsigned* __restrict__ order) {
     extern __shared__ att_pair_params s_tab[];
     const att_pair_params* tab = stage_table(par, s_tab);
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
-    const unsigned pose = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (pose >= n) return;
+    const unsigned wg = order ? xcd_slot(blockIdx.x, gridDim.x) : blockIdx.x;
+    const unsigned slot = wg * (blockDim.x / kWave) + threadIdx.x / kWave;
+    if (slot >= n) return;
+    const unsigned pose = order ? order[slot] : slot;
     const float* d = dofs + 6 * (size_t)pose;
     const float tx = d[0], ty = d[1], tz = d[2];
     float t[6];
@@ -77,7 +80,8 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
                                                DNL nl, DGrid lInner, DGrid lOuter, DPar par,
                                                const float* __restrict__ dofs,
                                                float* __restrict__ res, unsigned n,
-                                               unsigned tabSlots) {
+                                               unsigned tabSlots,
+                                               const unsigned* __restrict__ order) {
     extern __shared__ att_pair_params s_dyn[];
     __shared__ float s_red[4][kModesVals];
     const att_pair_params* tab = stage_table(par, s_dyn);
@@ -87,8 +91,8 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     unsigned short* s_rt = reinterpret_cast<unsigned short*>(s_rz + rec.n);
     unsigned* s_queue = reinterpret_cast<unsigned*>(s_rt + ((rec.n + 1) & ~1u));
     __shared__ unsigned s_qn;
-    const unsigned pose = blockIdx.x;
-    if (pose >= n) return;  // uniform per workgroup
+    if (blockIdx.x >= n) return;  // uniform per workgroup
+    const unsigned pose = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const float* d = dofs + 26 * (size_t)pose;
@@ -401,13 +405,13 @@ hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s) {
 
 hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, const DGrid& outer,
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
-                        hipStream_t s) {
+                        const unsigned* order, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const unsigned posesPerBlock = 4;
     const unsigned blocks = (n + posesPerBlock - 1) / posesPerBlock;
     const size_t lds = par.T <= kMaxTypesLDS ? sizeof(att_pair_params) * par.T * par.T : 0;
     hipLaunchKernelGGL(k_rigid, dim3(blocks), dim3(256), lds, s, rec, lig, inner, outer, nl, par,
-                       dofs, res, n);
+                       dofs, res, n, order);
     return hipGetLastError();
 }
 
@@ -424,20 +428,20 @@ size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsi
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
-                        unsigned n, hipStream_t s) {
+                        unsigned n, const unsigned* order, hipStream_t s) {
     if (n == 0) return hipSuccess;
     unsigned slots = 0;
     const size_t lds = modes_lds_bytes(rec, lig, par, &slots);
     const unsigned mm = rec.M > lig.M ? rec.M : lig.M;
     if (mm == 0)
         hipLaunchKernelGGL(k_modes<0>, dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, nl,
-                           lInner, lOuter, par, dofs, res, n, slots);
+                           lInner, lOuter, par, dofs, res, n, slots, order);
     else if (mm <= 5)
         hipLaunchKernelGGL(k_modes<5>, dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, nl,
-                           lInner, lOuter, par, dofs, res, n, slots);
+                           lInner, lOuter, par, dofs, res, n, slots, order);
     else
         hipLaunchKernelGGL(k_modes<ATT_MODES_MAX>, dim3(n), dim3(256), lds, s, rec, lig, rInner,
-                           rOuter, nl, lInner, lOuter, par, dofs, res, n, slots);
+                           rOuter, nl, lInner, lOuter, par, dofs, res, n, slots, order);
     return hipGetLastError();
 }
 

@@ -432,6 +432,12 @@ class Context:
         self._chk(self.L.att_get_stream(self.h, C.byref(s)))
         return s.value
 
+    def new_stream(self):
+        """A stream of the context's own (att_stream_create), destroyed with it."""
+        s = P()
+        self._chk(self.L.att_stream_create(self.h, C.byref(s)))
+        return s.value
+
     def event(self):
         e = P()
         self._chk(self.L.att_event_create(self.h, C.byref(e)))

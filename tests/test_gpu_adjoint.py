@@ -146,7 +146,7 @@ def test_spatial_pose_order_changes_no_pose_result(W):
     bits)."""
     from gpuattract_amd import hip
     # enough poses for the ordered path (at least four per resident workgroup)
-    wl = W.make_workload("order", 61, 600, 150, 6000, 3, 2, with_basis=True)
+    wl = W.make_workload("order", 61, 600, 150, 12000, 3, 2, with_basis=True)
     T = wl.params.T
     ctx = hip.Context(0)
     try:
@@ -161,5 +161,42 @@ def test_spatial_pose_order_changes_no_pose_result(W):
         c, pc = ctx.adjoint(hs, wl.dofs, pgrad=True, T=T)
         assert np.array_equal(a, c)
         assert np.allclose(pa, pc, rtol=1e-5, atol=1e-6 * np.abs(pc).max())
+    finally:
+        ctx.close()
+
+
+def test_concurrent_streams_take_turns_on_scratch(W):
+    """Two device-resident calls in flight on two streams of one context use
+    its scratch (pose order, parameter-gradient rows) in turn: each returns
+    what it returns alone."""
+    from gpuattract_amd import hip
+    wl = W.make_workload("streams", 62, 600, 150, 12000, 3, 2, with_basis=True)
+    T = wl.params.T
+    n, cols = wl.dofs.shape[0], 3 * T * T
+    ctx = hip.Context(0)
+    try:
+        hs = ctx.upload_workload(wl)
+        ctx.upload_basis(hs["grid"], wl)
+        sets = [wl.dofs, wl.dofs.copy()]
+        sets[1][:, :3] += np.float32(1.5)
+        d_in = [ctx.to_device(d) for d in sets]
+        d_out = [ctx.dev_alloc(n * 27 * 4) for _ in sets]
+        d_pg = [ctx.dev_alloc(cols * 4) for _ in sets]
+        alone = []
+        for k in range(2):
+            ctx.adjoint_dev(hs, d_in[k].ptr, True, n, None, d_out[k].ptr, d_pg[k].ptr)
+            ctx.sync()
+            alone.append((ctx.to_host(d_out[k], (n, 27)), ctx.to_host(d_pg[k], (cols,))))
+        streams = [ctx.new_stream(), ctx.new_stream()]
+        for k in range(2):
+            ctx.write(d_out[k], np.full((n, 27), -7.0, np.float32))
+        for k in range(2):
+            ctx.adjoint_dev(hs, d_in[k].ptr, True, n, None, d_out[k].ptr, d_pg[k].ptr, streams[k])
+        for k in range(2):
+            ctx.sync(streams[k])
+        for k in range(2):
+            out, pg = ctx.to_host(d_out[k], (n, 27)), ctx.to_host(d_pg[k], (cols,))
+            assert np.array_equal(out.view(np.uint32), alone[k][0].view(np.uint32))
+            assert np.allclose(pg, alone[k][1], rtol=1e-5, atol=1e-6 * np.abs(alone[k][1]).max())
     finally:
         ctx.close()

@@ -147,8 +147,8 @@ def test_spatial_order_changes_no_result(ctx, staged):
         b = _batch(ctx, staged)
         for k in range(len(b.buckets())):
             b.set_shape(k, 2, 4)
-        for s in staged:  # no slot keeps the previous run's value
-            ctx.write(s["d_out"], np.full((s["n"], s["ow"]), np.nan, np.float32))
+        for s in staged:  # different fills: a slot left unwritten by either run differs
+            ctx.write(s["d_out"], np.full((s["n"], s["ow"]), np.nan if enabled else -7.0, np.float32))
         ctx.sync()
         b.run()
         ctx.sync()
@@ -156,4 +156,4 @@ def test_spatial_order_changes_no_result(ctx, staged):
         b.close()
     ctx.set_pose_order(True)
     for a, c, s in zip(runs[0], runs[1], staged):
-        assert np.array_equal(a, c), s["wl"].name
+        assert np.array_equal(a.view(np.uint32), c.view(np.uint32)), s["wl"].name

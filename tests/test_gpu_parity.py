@@ -137,6 +137,33 @@ def test_device_resident_matches_host_path(ctx, wl_modes, wl_rigid):
         assert np.array_equal(ctx.to_host(d_out, (n, out_w)), host)
 
 
+def test_forward_spatial_order_changes_no_result(ctx, wl_modes, wl_rigid):
+    """From 4096 poses on, the forward kernels take their poses in spatial
+    order, the workgroups of each XCD one contiguous range of it; every pose
+    is still written to its own slot, so the outputs are the same bits as in
+    input order.  5003 poses: the ranges of the eight XCDs are uneven."""
+    rng = np.random.default_rng(3)
+    for wl, out_w in ((wl_rigid, 7), (wl_modes, 27)):
+        hs = ctx.upload_workload(wl)
+        n = 5003
+        d = np.resize(wl.dofs, (n, wl.dofs.shape[1])).astype(np.float32)
+        d[:, :3] += rng.normal(scale=2.0, size=(n, 3)).astype(np.float32)
+        d_in = ctx.to_device(d)
+        d_out = ctx.dev_alloc(n * out_w * 4)
+        f = ctx.score_modes_dev if wl.modes else ctx.score_rigid_dev
+        res = []
+        # different fills before the two runs: a slot left unwritten by either
+        # run differs (clashing poses may legitimately give inf or NaN)
+        for enabled, fill in ((False, np.nan), (True, -7.0)):
+            ctx.set_pose_order(enabled)
+            ctx.write(d_out, np.full((n, out_w), fill, np.float32))
+            f(hs, d_in.ptr, n, d_out.ptr, None)
+            ctx.sync()
+            res.append(ctx.to_host(d_out, (n, out_w)))
+        ctx.set_pose_order(True)
+        assert np.array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
 def test_split_and_permutation_invariance(ctx, wl_rigid):
     hs = ctx.upload_workload(wl_rigid)
     full = ctx.score_rigid(hs, wl_rigid.dofs)
