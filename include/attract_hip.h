@@ -11,7 +11,10 @@
  * Threading: one context per host thread (the reference binds one worker
  * thread per device, service/GPUEnergyService6D.tpp:469-477).  Calls on one
  * context must not overlap; several contexts may share a device.  `stream` is
- * a hipStream_t (NULL = the context's own stream).
+ * a hipStream_t (NULL = the context's own stream).  Device-resident calls on
+ * different streams of one context may run concurrently; where they use the
+ * context's device scratch (spatial pose order from 4096 poses on, adjoint
+ * parameter-gradient rows) they take turns on it in stream order.
  */
 #ifndef ATTRACT_HIP_H_
 #define ATTRACT_HIP_H_
