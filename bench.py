@@ -96,6 +96,8 @@ def main():
     ap.add_argument("--poses", type=int, default=100000)
     ap.add_argument("--cpu-poses", type=int, default=50000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-server", action="store_true",
+                    help="batched workload: skip timing the drop-in Server path pair by pair")
     ap.add_argument("--no-pgrad", action="store_true",
                     help="DOF and mode gradients only (no pair-parameter gradient)")
     ap.add_argument("--plain-poses", type=int, default=5000)
@@ -215,6 +217,17 @@ def main():
         host_score(hs, wl.dofs[a:a + 100])
     plain = m / (time.perf_counter() - t0)
     log(f"plain item-100 host path: {plain:.4g} poses/s")
+    # the drop-in path itself (as::Server, items of 100), best of 1 and 2 workers
+    ms = min(n, 20000)
+    server_by_workers = {}
+    for workers in (1, 2):
+        sec = server_pass_seconds(np, hip, wl, wl.dofs[:ms], workers)
+        if sec:
+            server_by_workers[workers] = ms / sec
+    server = max(server_by_workers.values()) if server_by_workers else None
+    if server:
+        log("drop-in Server path, items of 100, 8 requests in flight: " + ", ".join(
+            f"{w} worker(s) {r:.4g} poses/s" for w, r in server_by_workers.items()))
 
     cpu = None
     if not args.no_cpu:
@@ -250,6 +263,9 @@ def main():
             "forward_kernel_poses_per_s": n / fwd_ms * 1e3,
             "plain_item100_poses_per_s": plain,
             "adjoint_vs_plain": value / plain,
+            "server_item100_poses_per_s": server,
+            "server_item100_by_workers": server_by_workers,
+            "adjoint_vs_server": value / server if server else None,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -410,6 +426,81 @@ def run_serve(args, np, W):
     print(json.dumps(line), flush=True)
 
 
+_DRIVER = None
+
+
+def _driver():
+    global _DRIVER
+    if _DRIVER is None:
+        import ctypes as C
+        path = os.path.join(ROOT, "tests", "cpp", "_bin", "libservice_driver.so")
+        if not os.path.exists(path):
+            return None
+        _DRIVER = C.CDLL(path)
+        _DRIVER.drv_error.restype = C.c_char_p
+    return _DRIVER
+
+
+def server_pass_seconds(np, hip, wl, dofs, workers, requests=8, item=100, reps=3):
+    """Wall time of one pass of the drop-in path over `dofs`: as::Server with
+    `workers` workers and the MI355 scoring service of the pair (items of
+    `item` poses; 100 is the reference's default, server/Server.h:39), the
+    poses split into `requests` requests submitted together; mean of `reps`
+    timed passes after one untimed pass.  Driven through the C++ driver
+    (tests/cpp/service_driver.cpp); None when it is not built."""
+    import ctypes as C
+    L = _driver()
+    if L is None:
+        return None
+    p = wl.params
+    tab = hip.param_table(p)
+    rec, lig = hip.protein_desc(wl.rec), hip.protein_desc(wl.lig)
+    gi, go, nl = hip.grid_desc(wl.rec_inner), hip.grid_desc(wl.rec_outer), hip.nl_desc(wl.rec_nl)
+    d = np.ascontiguousarray(dofs, np.float32)
+    m = d.shape[0]
+    out = np.zeros((m, 27 if wl.modes else 7), np.float32)
+    tail = (hip._p(tab), p.T, p.shape, p.dielec, C.c_float(p.epsilon), C.c_float(p.ffelec),
+            hip._p(d), C.c_size_t(m), hip._p(out), workers, item, requests, reps)
+    sec = C.c_double(0.0)
+    if wl.modes:
+        li, lo = hip.grid_desc(wl.lig_inner), hip.grid_desc(wl.lig_outer)
+        st = L.drv_time_modes(C.byref(rec), C.byref(lig), C.byref(gi), C.byref(go), C.byref(nl),
+                              C.byref(li), C.byref(lo), *tail, C.byref(sec))
+    else:
+        st = L.drv_time_rigid(C.byref(rec), C.byref(lig), C.byref(gi), C.byref(go), C.byref(nl),
+                              *tail, C.byref(sec))
+    if st != 0:
+        raise RuntimeError("drop-in Server path: " + L.drv_error().decode())
+    return sec.value
+
+
+def server_rate_batched(np, hip, W, ctx, staged, total):
+    """The same pairs through the drop-in path, one Server per pair (2 workers,
+    items of 100 poses, 8 requests in flight), pair after pair; the ligand
+    grids the modes service needs are generated on the device here (the
+    batched adjoint does not use them).  Poses/s over the timed passes."""
+    t0 = time.perf_counter()
+    secs = 0.0
+    for k, s in enumerate(staged):
+        wl = s["wl"]
+        if wl.modes and wl.lig_inner is None:
+            gli, glo = W.pair_lig_grid_geoms(wl)
+            T = wl.params.T
+            wl.lig_inner = W.IGrid(gli, T + 1, ctx.generate_grid(wl.lig, wl.params, gli, src_is_ligand=True))
+            wl.lig_outer = W.IGrid(glo, T + 1, ctx.generate_grid(wl.lig, wl.params, glo, src_is_ligand=True))
+        sec = server_pass_seconds(np, hip, wl, wl.dofs, 2, reps=1)
+        if wl.modes:
+            wl.lig_inner = wl.lig_outer = None  # host memory: one pair's grids at a time
+        if sec is None:
+            return None
+        secs += sec
+        if k % 16 == 15:
+            log(f"  drop-in Server path: {k + 1} pairs ({time.perf_counter() - t0:.0f} s)")
+    rate = total / secs
+    log(f"drop-in Server path (2 workers, items of 100), pair after pair: {rate:.4g} poses/s")
+    return rate
+
+
 def run_batched(args, np, hip, traffic, W):
     """Uneven pairs (receptor 500-5000 beads, ligand 100-1500, 0-10 modes per
     body, 1000-50000 poses, all log-uniform except modes) scored by the batched
@@ -471,6 +562,7 @@ def run_batched(args, np, hip, traffic, W):
         done += d.shape[0]
     plain = done / (time.perf_counter() - t0)
     log(f"per-pair device launches: {per_pair:.4g} poses/s; plain item-100 path: {plain:.4g} poses/s")
+    server = None if args.no_server else server_rate_batched(np, hip, W, ctx, staged, total)
 
     bpp = [traffic.bytes_per_pose(s["wl"], sample=64, adjoint=True)[0] for s in staged]
     algo = sum(bp * s["n"] for bp, s in zip(bpp, staged))
@@ -501,6 +593,8 @@ def run_batched(args, np, hip, traffic, W):
                    "poses": total, "path": "batched adjoint, autotuned buckets, hipGraph replay",
                    "buckets": buckets, "per_pair_launches_poses_per_s": per_pair,
                    "plain_item100_poses_per_s": plain, "batched_vs_plain": value / plain,
+                   "server_item100_poses_per_s": server,
+                   "batched_vs_server": value / server if server else None,
                    "batched_vs_per_pair": value / per_pair},
         "roofline": roofline, "cpu_baseline": cpu,
     }

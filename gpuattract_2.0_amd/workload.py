@@ -295,6 +295,21 @@ def scaled_radius(n, ref):
     return ref[1] * (n / ref[0]) ** (1.0 / 3.0)
 
 
+def lig_grid_geoms(lig, R_rec, r_hi, spacing=0.875, margin=10.0, outer_spacing=2.6):
+    """Inner and outer geometry of a ligand's grids (the receptor beads are
+    looked up in them): the inner box spans the ligand plus the margin, the
+    outer cube every receptor bead of a pose whose translation is at most r_hi."""
+    return (inner_geom(lig, spacing, margin),
+            cube_geom((0.0, 0.0, 0.0), r_hi + R_rec + 2.0, outer_spacing))
+
+
+def pair_lig_grid_geoms(wl):
+    """lig_grid_geoms of a pair built by make_workload with its default shell."""
+    R_rec = scaled_radius(wl.rec.n, DENSITY_REF)
+    R_lig = scaled_radius(wl.lig.n, LIG_REF)
+    return lig_grid_geoms(wl.lig, R_rec, R_rec + R_lig)
+
+
 def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_types=12,
                   spacing=0.875, margin=10.0, outer_spacing=2.6, plateau=10.0,
                   neighbourdis=10.8, charged_frac=0.2, type0_frac=0.0, shell=None,
@@ -323,9 +338,8 @@ def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_ty
     dofs6 = make_poses(seed, n_poses, r_lo, r_hi)
     wl = Workload(name, rec, lig, params, rec_inner, rec_outer, rec_nl, dofs6)
     if modes:
-        gli = inner_geom(lig, spacing, margin)
+        gli, glo = lig_grid_geoms(lig, R_rec, r_hi, spacing, margin, outer_spacing)
         wl.lig_inner = build_grid(gli, lig, params, True, plateau)
-        glo = cube_geom((0.0, 0.0, 0.0), r_hi + R_rec + 2.0, outer_spacing)
         wl.lig_outer = build_grid(glo, lig, params, True, plateau)
         wl.lig_nl = build_nl(gli, lig, neighbourdis, plateau)
     if modes or M_rec > 0 or M_lig > 0:

@@ -1,12 +1,14 @@
-// Test driver (not part of the library): runs scoring requests through
-// as::Server with the MI355 services, so the GPU tests can compare the
-// drop-in path with direct C-ABI calls on the same workloads.
+// Test and benchmark driver (not part of the library): runs scoring requests
+// through as::Server with the MI355 services, so the GPU tests can compare the
+// drop-in path with direct C-ABI calls on the same workloads and the benchmark
+// can time the drop-in path itself (drv_time_rigid / drv_time_modes).
 //
 // drv_score_rigid / drv_score_modes copy the descriptors' arrays into a
 // DataManager, create the service on HIP device 0 with `workers` workers and
 // the given item size, split the poses into `requests` requests submitted
 // together, wait for them in reverse order and write the results in input
 // order.  Return 0, or -1 with drv_error() describing the failure.
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -77,6 +79,36 @@ int run(std::shared_ptr<Service> svc, Common common, const typename Service::inp
     return 0;
 }
 
+// One Server with `workers` workers and the given item size scores the n
+// poses as `requests` requests submitted together, once untimed, then `reps`
+// times; returns the mean wall time of one timed pass (submit of every
+// request to the last wait).
+template <class Service, class Common>
+double time_server(std::shared_ptr<Service> svc, Common common, const typename Service::input_t* dofs,
+                   size_t n, typename Service::result_t* out, unsigned workers, unsigned itemSize,
+                   unsigned requests, unsigned reps) {
+    using server_t = as::Server<typename Service::genericTypes_t>;
+    server_t server(svc);
+    server.createWorkers(workers);
+    server.setItemSize(itemSize);
+    std::vector<typename Service::input_t> in(dofs, dofs + n);
+    std::vector<typename server_t::request_t> reqs(requests);
+    std::vector<size_t> lo(requests + 1);
+    for (unsigned r = 0; r <= requests; ++r) lo[r] = n * r / requests;
+    auto pass = [&]() {
+        for (unsigned r = 0; r < requests; ++r) {
+            reqs[r].setDataAndSize(in.data() + lo[r], lo[r + 1] - lo[r]);
+            reqs[r].setCommon(common);
+            server.submit(reqs[r]);
+        }
+        for (unsigned r = 0; r < requests; ++r) server.wait(reqs[r], out + lo[r]);
+    };
+    pass();
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 0; k < reps; ++k) pass();
+    const auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count() / (reps ? reps : 1);
+}
 std::shared_ptr<as::DataManager> tables(std::shared_ptr<as::DataManager> dm, const att_pair_params* tab,
                                         unsigned T, int shape, int dielec, float eps, float ffelec,
                                         as::id_t& tableId, as::id_t& paramsId) {
@@ -135,6 +167,55 @@ int drv_score_modes(const att_protein_desc* rec, const att_protein_desc* lig, co
         auto svc = std::make_shared<as::MI355EnergyService6DModes<float>>(dm, std::vector<int>{0});
         return run(svc, c, reinterpret_cast<const as::DOF_6D_Modes<float>*>(dofs), n,
                    reinterpret_cast<as::Result_6D_Modes<float>*>(out), workers, itemSize, requests);
+    } catch (std::exception const& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+
+/* Throughput of the drop-in path (time_server above): *seconds receives the
+ * mean wall time of one pass over the n poses. */
+int drv_time_rigid(const att_protein_desc* rec, const att_protein_desc* lig, const att_grid_desc* inner,
+                   const att_grid_desc* outer, const att_nlgrid_desc* nl, const att_pair_params* tab,
+                   unsigned T, int shape, int dielec, float eps, float ffelec, const float* dofs,
+                   size_t n, float* out, unsigned workers, unsigned itemSize, unsigned requests,
+                   unsigned reps, double* seconds) {
+    try {
+        auto dm = std::make_shared<as::DataManager>();
+        as::Common c{};
+        c.recId = dm->add(protein(*rec));
+        c.ligId = dm->add(protein(*lig));
+        c.gridId = dm->add(grid_union(*inner, *outer, nl));
+        tables(dm, tab, T, shape, dielec, eps, ffelec, c.tableId, c.paramsId);
+        auto svc = std::make_shared<as::MI355EnergyService6D<float>>(dm, std::vector<int>{0});
+        *seconds = time_server(svc, c, reinterpret_cast<const as::DOF_6D<float>*>(dofs), n,
+                               reinterpret_cast<as::Result_6D<float>*>(out), workers, itemSize,
+                               requests, reps);
+        return 0;
+    } catch (std::exception const& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+int drv_time_modes(const att_protein_desc* rec, const att_protein_desc* lig, const att_grid_desc* rInner,
+                   const att_grid_desc* rOuter, const att_nlgrid_desc* nl, const att_grid_desc* lInner,
+                   const att_grid_desc* lOuter, const att_pair_params* tab, unsigned T, int shape,
+                   int dielec, float eps, float ffelec, const float* dofs, size_t n, float* out,
+                   unsigned workers, unsigned itemSize, unsigned requests, unsigned reps,
+                   double* seconds) {
+    try {
+        auto dm = std::make_shared<as::DataManager>();
+        as::Common_Modes c{};
+        c.recId = dm->add(protein(*rec));
+        c.ligId = dm->add(protein(*lig));
+        c.gridIdRec = dm->add(grid_union(*rInner, *rOuter, nl));
+        c.gridIdLig = dm->add(grid_union(*lInner, *lOuter, nullptr));
+        tables(dm, tab, T, shape, dielec, eps, ffelec, c.tableId, c.paramsId);
+        auto svc = std::make_shared<as::MI355EnergyService6DModes<float>>(dm, std::vector<int>{0});
+        *seconds = time_server(svc, c, reinterpret_cast<const as::DOF_6D_Modes<float>*>(dofs), n,
+                               reinterpret_cast<as::Result_6D_Modes<float>*>(out), workers, itemSize,
+                               requests, reps);
+        return 0;
     } catch (std::exception const& e) {
         g_error = e.what();
         return -1;
