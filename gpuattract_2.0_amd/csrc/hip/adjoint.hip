@@ -61,9 +61,10 @@ struct PGrad {
 };
 
 // Basis channels read per step of the parameter-gradient loop: the loads of
-// kBasisStep channels are issued together (64 B each), so a bead waits for
-// ceil(T / kBasisStep) memory round trips instead of T.
-constexpr unsigned kBasisStep = 4;
+// kBasisStep channels are issued together (64 B each) and their LDS updates
+// follow.  Timed with 12 types: steps of 2 and 3 channels 1.1% faster than 4
+// and 6 (same results).
+constexpr unsigned kBasisStep = 2;
 
 // Grid term of one ligand bead on one grid: energy (the forward's per-bead
 // value), gradient, and the basis-grid parameter derivatives.
