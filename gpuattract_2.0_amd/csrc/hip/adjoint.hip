@@ -21,7 +21,8 @@
 // k_adjoint: persistent, one 256-thread workgroup per pose at a time; per-pose
 // values go through wave butterflies, parameter derivatives accumulate in LDS
 // across the workgroup's poses and are written once per workgroup, then
-// k_pgrad_reduce adds the per-workgroup rows in a fixed order.
+// k_pgrad_groups and k_pgrad_reduce add the per-workgroup rows in a fixed
+// order.
 #include <algorithm>
 
 #include "device.h"
@@ -622,22 +623,37 @@ hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Sums the per-workgroup parameter-gradient rows in block order.
-__global__ void k_pgrad_reduce(const float* __restrict__ partial, unsigned rows, unsigned cols,
+// Parameter-gradient rows of the workgroups, summed in a fixed order: group
+// g (blockIdx.y) adds rows [g*rows/G, (g+1)*rows/G) in order, in double.
+__global__ void k_pgrad_groups(const float* __restrict__ partial, unsigned rows, unsigned cols,
+                               double* __restrict__ groups) {
+    const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    const unsigned g = blockIdx.y;
+    const unsigned lo = (unsigned)((unsigned long long)rows * g / kPgradGroups);
+    const unsigned hi = (unsigned)((unsigned long long)rows * (g + 1) / kPgradGroups);
+    // the loads of 8 rows are issued together
+    double s = 0.0;
+    unsigned r = lo;
+    for (; r + 8 <= hi; r += 8) {
+        float v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[(size_t)(r + k) * cols + c];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += v[k];
+    }
+    for (; r < hi; ++r) s += partial[(size_t)r * cols + c];
+    groups[(size_t)g * cols + c] = s;
+}
+
+// The group sums added in group order.
+__global__ void k_pgrad_reduce(const double* __restrict__ groups, unsigned cols,
                                float* __restrict__ pg) {
     const unsigned c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols) return;
-    // rows are added in order; the loads of 16 rows are issued together
     double s = 0.0;
-    unsigned r = 0;
-    for (; r + 16 <= rows; r += 16) {
-        float v[16];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) v[k] = partial[(size_t)(r + k) * cols + c];
-#pragma unroll
-        for (int k = 0; k < 16; ++k) s += v[k];
-    }
-    for (; r < rows; ++r) s += partial[(size_t)r * cols + c];
+#pragma unroll 8
+    for (unsigned g = 0; g < kPgradGroups; ++g) s += groups[(size_t)g * cols + c];
     pg[c] = (float)s;
 }
 
@@ -722,10 +738,12 @@ hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, c
                            lds, s);
 }
 
-hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, float* pg,
-                               hipStream_t s) {
-    hipLaunchKernelGGL(k_pgrad_reduce, dim3((cols + 255) / 256), dim3(256), 0, s, partial, rows,
-                       cols, pg);
+hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, double* groups,
+                               float* pg, hipStream_t s) {
+    hipLaunchKernelGGL(k_pgrad_groups, dim3((cols + 255) / 256, kPgradGroups), dim3(256), 0, s,
+                       partial, rows, cols, groups);
+    hipLaunchKernelGGL(k_pgrad_reduce, dim3((cols + 255) / 256), dim3(256), 0, s,
+                       (const double*)groups, cols, pg);
     return hipGetLastError();
 }
 

@@ -80,8 +80,13 @@ hipError_t launch_min_step(const MinRun& R, const unsigned* listIn, const unsign
 // Zero-fills n floats with a kernel (graph-capturable, unlike hipMemsetAsync).
 hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s);
 
-hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, float* pg,
-                               hipStream_t s);
+// Sums rows x cols per-workgroup parameter rows into pg in a fixed order:
+// kPgradGroups groups of consecutive rows in double (into `groups`, scratch of
+// pgrad_group_bytes(cols)), then the group sums in order.
+constexpr unsigned kPgradGroups = 32;
+inline size_t pgrad_group_bytes(unsigned cols) { return sizeof(double) * kPgradGroups * cols; }
+hipError_t launch_pgrad_reduce(const float* partial, unsigned rows, unsigned cols, double* groups,
+                               float* pg, hipStream_t s);
 
 hipError_t launch_basis_octets(const float* b, float4* oct, unsigned nx, unsigned ny, unsigned nz,
                                unsigned nb, hipStream_t s);

@@ -677,7 +677,9 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
     const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
-    if (pgrad && (st = grow(c, &c->d_pg, &c->d_pg_cap, sizeof(float) * (size_t)blocks * cols)) != ATT_OK)
+    // per-workgroup rows, then the reduction's group sums (8-byte aligned)
+    const size_t rowBytes = (sizeof(float) * (size_t)blocks * cols + 255) & ~(size_t)255;
+    if (pgrad && (st = grow(c, &c->d_pg, &c->d_pg_cap, rowBytes + pgrad_group_bytes(cols))) != ATT_OK)
         return st;
     if (n == 0) {
         if (pgrad) ATT_HIP(c, launch_fill_zero(d_pgrad, cols, s));
@@ -698,7 +700,9 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     ATT_HIP(c, launch_adjoint(P, modes != 0, pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks,
                               kWaves, dyn, s));
     if (pgrad)
-        ATT_HIP(c, launch_pgrad_reduce(static_cast<const float*>(c->d_pg), blocks, cols, d_pgrad, s));
+        ATT_HIP(c, launch_pgrad_reduce(static_cast<const float*>(c->d_pg), blocks, cols,
+                                       reinterpret_cast<double*>(static_cast<char*>(c->d_pg) + rowBytes),
+                                       d_pgrad, s));
     if (scratch) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
