@@ -564,7 +564,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
         end = (unsigned)(((unsigned long long)n * (x + 1)) >> 3);
     }
     for (unsigned i = first; i < end; i += step)
-        adjoint_pose<MM, WAVES>(P, ordered ? P.order[i] : i, S, dyn, tab, s_pg);
+        adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, dyn, tab, s_pg);
     if (s_pg) {
         __syncthreads();
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x)
