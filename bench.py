@@ -97,7 +97,7 @@ def main():
     ap.add_argument("--cpu-poses", type=int, default=50000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-server", action="store_true",
-                    help="batched workload: skip timing the drop-in Server path pair by pair")
+                    help="skip timing the drop-in Server path")
     ap.add_argument("--no-pgrad", action="store_true",
                     help="DOF and mode gradients only (no pair-parameter gradient)")
     ap.add_argument("--plain-poses", type=int, default=5000)
@@ -220,7 +220,7 @@ def main():
     # the drop-in path itself (as::Server, items of 100), best of 1 and 2 workers
     ms = min(n, 20000)
     server_by_workers = {}
-    for workers in (1, 2):
+    for workers in (() if args.no_server else (1, 2)):
         sec = server_pass_seconds(np, hip, wl, wl.dofs[:ms], workers)
         if sec:
             server_by_workers[workers] = ms / sec
