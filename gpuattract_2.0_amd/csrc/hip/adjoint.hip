@@ -203,6 +203,47 @@ __device__ __forceinline__ unsigned block_scan(unsigned v, unsigned* s_w, unsign
     return before + x - v;
 }
 
+// Lane i reads lane i + O of its row of 16 (DPP row shift; O = 1, 2, 4, 8):
+// no LDS traffic, unlike a cross-lane shuffle.
+template <int O>
+__device__ __forceinline__ unsigned row_down(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x100 + O, 0xF, 0xF, false);
+}
+template <int O>
+__device__ __forceinline__ float row_down(float v) {
+    return __uint_as_float(row_down<O>(__float_as_uint(v)));
+}
+
+// Segmented sums over runs of lanes with equal `seg` (runs are contiguous:
+// seg never decreases across the wave): the first lane of each run receives
+// the sum of the run, in a fixed order.  First the suffix sums inside each
+// row of 16 lanes (DPP), then the partial of every later row whose first lane
+// continues the run, taken from those lanes in row order.
+__device__ __forceinline__ void seg_run_sums(float& Fx, float& Fy, float& Fz, unsigned seg,
+                                             unsigned lane) {
+    const unsigned col = lane & 15u;
+#define ATT_ROW_STEP(O)                                                              \
+    {                                                                                \
+        const float ox = row_down<O>(Fx), oy = row_down<O>(Fy), oz = row_down<O>(Fz); \
+        const unsigned os = row_down<O>(seg);                                        \
+        if (col + O < 16u && os == seg) { Fx += ox; Fy += oy; Fz += oz; }            \
+    }
+    ATT_ROW_STEP(1)
+    ATT_ROW_STEP(2)
+    ATT_ROW_STEP(4)
+    ATT_ROW_STEP(8)
+#undef ATT_ROW_STEP
+    const unsigned row = lane >> 4;
+#pragma unroll
+    for (unsigned r = 1; r < 4; ++r) {
+        const unsigned hs = (unsigned)__builtin_amdgcn_readlane((int)seg, 16 * r);
+        const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fx), 16 * r));
+        const float hy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fy), 16 * r));
+        const float hz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fz), 16 * r));
+        if (row < r && hs == seg) { Fx += hx; Fy += hy; Fz += hz; }
+    }
+}
+
 // Workgroup-shared state of one pose (static LDS); WAVES waves per pose,
 // 64 * WAVES queued ligand beads per pair round.
 template <int WAVES>
@@ -454,12 +495,7 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                 }
                 // sum of the forces of each run of lanes sharing a slot, in its first lane
                 const unsigned seg = valid ? s : 0xFFFFFFFFu;
-#pragma unroll
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const float ox = __shfl_down(Fx, o), oy = __shfl_down(Fy, o), oz = __shfl_down(Fz, o);
-                    const unsigned os = __shfl_down(seg, o);
-                    if (lane + o < (unsigned)kWave && os == seg) { Fx += ox; Fy += oy; Fz += oz; }
-                }
+                seg_run_sums(Fx, Fy, Fz, seg, lane);
                 const unsigned prev = __shfl_up(seg, 1);
                 if (valid && (lane == 0 || prev != seg)) {
                     if (headSplit && seg == firstSlot) {
