@@ -203,6 +203,33 @@ __device__ __forceinline__ unsigned block_scan(unsigned v, unsigned* s_w, unsign
     return before + x - v;
 }
 
+// OR of v over the lanes of each row of 16 (DPP row rotations, which need no
+// shift direction), then over the four rows (row heads by readlane): uniform.
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+#define ATT_ROR_OR(O) v |= (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x120 + O, 0xF, 0xF, false);
+    ATT_ROR_OR(1)
+    ATT_ROR_OR(2)
+    ATT_ROR_OR(4)
+    ATT_ROR_OR(8)
+#undef ATT_ROR_OR
+    return (unsigned)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
+                      __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
+}
+
+// Lane j holds e_j, the end of the j-th slot of a chunk of 64 pairs starting
+// at B (ends strictly increasing, all above B; past the last slot, ~0u).
+// Returns, for the pair B + lane, the number of ends at or below it: the
+// ends inside (B, B + 64) are marked in one 64-bit mask (bit e_j - B), and
+// the lane counts the marks at or below its position.
+__device__ __forceinline__ unsigned ends_up_to(unsigned e, unsigned B, unsigned lane) {
+    const unsigned r = e - B;
+    const unsigned mlo = wave_or(r < 32u ? 1u << r : 0u);
+    const unsigned mhi = wave_or(r - 32u < 32u ? 1u << (r - 32u) : 0u);
+    const unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
+    const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
+    return (unsigned)__popcll(m & upto);
+}
+
 // Lane i reads lane i + O of its row of 16 (DPP row shift; O = 1, 2, 4, 8):
 // no LDS traffic, unlike a cross-lane shuffle.
 template <int O>
@@ -409,21 +436,9 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                 const bool valid = p < re;
                 // Slot of every lane: lane j holds the end of slot b0 + j (ends
                 // increase strictly, the first exceeds B); the lane's slot is b0 +
-                // the number of those ends at or below its pair, found by a binary
-                // search across the lanes.
+                // the number of those ends at or below its pair (ends_up_to).
                 const unsigned e = b0 + lane < ns ? S.end[b0 + lane] : 0xFFFFFFFFu;
-                // (a range of 64 candidates takes 7 halvings; every lane takes
-                // part in each shuffle, the source lane clamped to the wave)
-                unsigned lo = 0, hi = kWave;
-#pragma unroll
-                for (int step = 0; step < 7; ++step) {
-                    const unsigned mid = (lo + hi) >> 1;
-                    const unsigned em = __shfl(e, (int)(mid < (unsigned)kWave ? mid : kWave - 1));
-                    if (lo < hi) {
-                        if (em <= p) lo = mid + 1; else hi = mid;
-                    }
-                }
-                const unsigned s = b0 + lo;
+                const unsigned s = min(b0 + ends_up_to(e, B, lane), ns - 1u);
                 b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
 
                 float Fx = 0.f, Fy = 0.f, Fz = 0.f;
