@@ -219,13 +219,15 @@ __device__ __forceinline__ unsigned wave_or(unsigned v) {
 // Lane j holds e_j, the end of the j-th slot of a chunk of 64 pairs starting
 // at B (ends strictly increasing, all above B; past the last slot, ~0u).
 // Returns, for the pair B + lane, the number of ends at or below it: the
-// ends inside (B, B + 64) are marked in one 64-bit mask (bit e_j - B), and
-// the lane counts the marks at or below its position.
-__device__ __forceinline__ unsigned ends_up_to(unsigned e, unsigned B, unsigned lane) {
+// ends inside (B, B + 64) are marked in one 64-bit mask m (bit e_j - B: a
+// slot starts at that lane), and the lane counts the marks at or below its
+// position.
+__device__ __forceinline__ unsigned ends_up_to(unsigned e, unsigned B, unsigned lane,
+                                               unsigned long long& m) {
     const unsigned r = e - B;
     const unsigned mlo = wave_or(r < 32u ? 1u << r : 0u);
     const unsigned mhi = wave_or(r - 32u < 32u ? 1u << (r - 32u) : 0u);
-    const unsigned long long m = ((unsigned long long)mhi << 32) | mlo;
+    m = ((unsigned long long)mhi << 32) | mlo;
     const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
     return (unsigned)__popcll(m & upto);
 }
@@ -438,7 +440,8 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                 // increase strictly, the first exceeds B); the lane's slot is b0 +
                 // the number of those ends at or below its pair (ends_up_to).
                 const unsigned e = b0 + lane < ns ? S.end[b0 + lane] : 0xFFFFFFFFu;
-                const unsigned s = min(b0 + ends_up_to(e, B, lane), ns - 1u);
+                unsigned long long starts;
+                const unsigned s = min(b0 + ends_up_to(e, B, lane, starts), ns - 1u);
                 b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
 
                 float Fx = 0.f, Fy = 0.f, Fz = 0.f;
@@ -511,8 +514,8 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                 // sum of the forces of each run of lanes sharing a slot, in its first lane
                 const unsigned seg = valid ? s : 0xFFFFFFFFu;
                 seg_run_sums(Fx, Fy, Fz, seg, lane);
-                const unsigned prev = __shfl_up(seg, 1);
-                if (valid && (lane == 0 || prev != seg)) {
+                // the first lane of a run: lane 0, or a slot starts at the lane
+                if (valid && (lane == 0 || ((starts >> lane) & 1ull))) {
                     if (headSplit && seg == firstSlot) {
                         float4 h = S.head[wave];
                         h.x += Fx; h.y += Fy; h.z += Fz;
