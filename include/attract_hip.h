@@ -93,11 +93,13 @@ typedef struct { att_result6 r6; float modesRec[ATT_MODES_MAX]; float modesLig[A
  * Replaces the per-worker CUDA state of GPUEnergyService6D::Private
  * (service/GPUEnergyService6D.tpp:94-115) and cudaSetDevice per item (:474). */
 att_status att_ctx_create(int hip_device, att_ctx** out);
-/* Spatial pose order of the single-pair adjoint (default on): poses are
- * sorted by the Morton code of their translation and each XCD scores a
- * contiguous range of that order, so its L2 holds the grid lines its poses
- * share.  Per-pose results do not depend on it; the parameter-gradient sum
- * runs in a different (still fixed) order. */
+/* Spatial pose order (default on) of the single-pair adjoint (when every
+ * resident workgroup gets at least four poses), of the forward scoring calls
+ * from 4096 poses on, and of the pairs of a batch created while it is on
+ * (pairs of 256 poses or more): poses are sorted by the Morton code of their
+ * translation and each XCD scores a contiguous range of that order, so its
+ * L2 holds the grid lines its poses share.  Per-pose results do not depend on
+ * it; the parameter-gradient sum runs in a different (still fixed) order. */
 att_status att_set_pose_order(att_ctx* ctx, int enabled);
 void att_ctx_destroy(att_ctx* ctx);
 const char* att_last_error(const att_ctx* ctx);
