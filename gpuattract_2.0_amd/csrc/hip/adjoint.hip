@@ -663,6 +663,22 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
     }
 }
 
+// The wave primitives of the pair loop on given inputs (one wave, diagnostics).
+__global__ __launch_bounds__(64) void k_debug_wave_ops(const float* __restrict__ F,
+                                                       const unsigned* __restrict__ seg,
+                                                       const unsigned* __restrict__ ends, unsigned B,
+                                                       float* __restrict__ sums,
+                                                       unsigned* __restrict__ counts,
+                                                       unsigned long long* __restrict__ starts) {
+    const unsigned lane = threadIdx.x;
+    float Fx = F[lane], Fy = F[64 + lane], Fz = F[128 + lane];
+    seg_run_sums(Fx, Fy, Fz, seg[lane], lane);
+    sums[lane] = Fx; sums[64 + lane] = Fy; sums[128 + lane] = Fz;
+    unsigned long long m;
+    counts[lane] = ends_up_to(ends[lane], B, lane, m);
+    if (lane == 0) starts[0] = m;
+}
+
 __global__ void k_reset_counters(unsigned* counters, unsigned n) {
     if (threadIdx.x < n) counters[threadIdx.x] = 0u;
 }
@@ -779,6 +795,14 @@ hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsig
     void* args[] = {const_cast<PairDesc*>(&P), &pgPartial};
     return hipLaunchKernel(adjoint_kernel_for(mm, waves, false), dim3(blocks), dim3(64 * waves),
                            args, lds, s);
+}
+
+hipError_t launch_debug_wave_ops(const float* F, const unsigned* seg, const unsigned* ends,
+                                 unsigned B, float* sums, unsigned* counts,
+                                 unsigned long long* starts, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_wave_ops, dim3(1), dim3(64), 0, s, F, seg, ends, B, sums, counts,
+                       starts);
+    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,

@@ -77,6 +77,11 @@ hipError_t launch_min_start(const MinRun& R, const float* dofs, hipStream_t s);
 hipError_t launch_min_step(const MinRun& R, const unsigned* listIn, const unsigned* countIn,
                            const float* reqIn, const float* res, hipStream_t s);
 
+// One wave: the adjoint's segmented run sums and slot counts (diagnostics).
+hipError_t launch_debug_wave_ops(const float* F, const unsigned* seg, const unsigned* ends,
+                                 unsigned B, float* sums, unsigned* counts,
+                                 unsigned long long* starts, hipStream_t s);
+
 // Zero-fills n floats with a kernel (graph-capturable, unlike hipMemsetAsync).
 hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s);
 

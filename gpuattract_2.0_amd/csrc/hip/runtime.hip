@@ -828,6 +828,33 @@ att_status att_debug_rotmats(att_ctx* c, const float* angles, size_t n, float* R
     return ATT_OK;
 }
 
+att_status att_debug_wave_ops(att_ctx* c, const float* F, const unsigned* seg, const unsigned* ends,
+                              unsigned B, float* sums, unsigned* counts, uint64_t* starts) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!F || !seg || !ends || !sums || !counts || !starts)
+        return fail(c, ATT_ERR_INVALID, "null argument");
+    ATT_HIP(c, hipSetDevice(c->device));
+    Tmp t;
+    float *dF, *dS;
+    unsigned *dSeg, *dE, *dC;
+    unsigned long long* dM;
+    ATT_HIP(c, t.alloc(&dF, 192));
+    ATT_HIP(c, t.alloc(&dS, 192));
+    ATT_HIP(c, t.alloc(&dSeg, 64));
+    ATT_HIP(c, t.alloc(&dE, 64));
+    ATT_HIP(c, t.alloc(&dC, 64));
+    ATT_HIP(c, t.alloc(&dM, 1));
+    ATT_HIP(c, hipMemcpy(dF, F, sizeof(float) * 192, hipMemcpyHostToDevice));
+    ATT_HIP(c, hipMemcpy(dSeg, seg, sizeof(unsigned) * 64, hipMemcpyHostToDevice));
+    ATT_HIP(c, hipMemcpy(dE, ends, sizeof(unsigned) * 64, hipMemcpyHostToDevice));
+    ATT_HIP(c, launch_debug_wave_ops(dF, dSeg, dE, B, dS, dC, dM, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    ATT_HIP(c, hipMemcpy(sums, dS, sizeof(float) * 192, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(counts, dC, sizeof(unsigned) * 64, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(starts, dM, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return ATT_OK;
+}
+
 att_status att_sync(att_ctx* c, void* stream) {
     if (!c) return ATT_ERR_INVALID;
     ATT_HIP(c, hipSetDevice(c->device));

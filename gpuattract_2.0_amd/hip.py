@@ -125,6 +125,7 @@ def lib():
         L.att_debug_beads_rigid.argtypes = [P, H, H, H, H, P, P, P, P, P]
         L.att_debug_beads_modes.argtypes = [P, H, H, H, H, H, P, P, P, P, P, P, P]
         L.att_debug_rotmats.argtypes = [P, P, C.c_size_t, P]
+        L.att_debug_wave_ops.argtypes = [P, P, P, P, C.c_uint, P, P, P]
         L.att_sync.argtypes = [P, P]
         L.att_build_info.restype = C.c_char_p
         L.att_dev_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
@@ -470,6 +471,19 @@ class Context:
                 _p(out["pot"]), _p(out["cell"]), _p(out["nl_pairs"])))
         out["cell"] = out["cell"].reshape(n, 9)
         return out
+
+    def wave_ops(self, F, seg, ends, B):
+        """The adjoint's wave primitives on one wave (att_debug_wave_ops):
+        (run sums (3, 64), counts (64,), starts mask)."""
+        f = np.ascontiguousarray(F, np.float32).reshape(3, 64)
+        sg = np.ascontiguousarray(seg, np.uint32).reshape(64)
+        e = np.ascontiguousarray(ends, np.uint32).reshape(64)
+        sums = np.zeros((3, 64), np.float32)
+        counts = np.zeros(64, np.uint32)
+        starts = np.zeros(1, np.uint64)
+        self._chk(self.L.att_debug_wave_ops(self.h, _p(f), _p(sg), _p(e), int(B), _p(sums),
+                                            _p(counts), _p(starts)))
+        return sums, counts, int(starts[0])
 
     def rotmats(self, angles):
         a = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)

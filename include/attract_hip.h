@@ -222,6 +222,17 @@ att_status att_debug_beads_modes(att_ctx* ctx, att_handle rec, att_handle lig, a
  * them (matrixFunctions.h:38-77), for checking them against the host. */
 att_status att_debug_rotmats(att_ctx* ctx, const float* angles /* 3n */, size_t n, float* R /* 9n */);
 
+/* The adjoint's wave primitives on one wave of 64 lanes, for checking them
+ * against the host: per lane a force F (SoA: x[64], y[64], z[64]) and a run key
+ * seg[64] (non-decreasing); sums (SoA 3x64) receives, in the first lane of
+ * every run of equal keys, the sum of the run's forces.  With ends[64] the
+ * strictly increasing slot ends of a chunk of 64 pairs starting at B (all
+ * above B; ~0u past the last slot), counts[64] receives for each lane the
+ * number of ends at or below B + lane and *starts the 64-bit mask of ends
+ * inside (B, B + 64).  No reference counterpart. */
+att_status att_debug_wave_ops(att_ctx* ctx, const float* F, const unsigned* seg, const unsigned* ends,
+                              unsigned B, float* sums, unsigned* counts, uint64_t* starts);
+
 att_status att_sync(att_ctx* ctx, void* stream);
 
 /* A plain float4 copy of `bytes` (multiple of 16) from d_src to d_dst on
