@@ -487,6 +487,129 @@ __host__ __device__ inline size_t adjoint_pg_offset(unsigned tabSlots, unsigned 
     return adjoint_queue_offset(tabSlots, ligN) + 2 * (((size_t)ligN + 1) & ~(size_t)1);
 }
 
+// Exclusive prefix sum of one value per thread over the workgroup (s_w holds
+// one value per wave); `total` receives the sum.  Contains two barriers.
+__device__ __forceinline__ unsigned block_scan(unsigned v, unsigned* s_w, unsigned& total) {
+    const unsigned lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    unsigned x = v;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned y = __shfl_up(x, d);
+        if (lane >= (unsigned)d) x += y;
+    }
+    if (lane == kWave - 1) s_w[wave] = x;
+    __syncthreads();
+    unsigned before = 0, tot = 0;
+    const unsigned nw = blockDim.x / kWave;
+    for (unsigned w = 0; w < nw; ++w) {
+        const unsigned t = s_w[w];
+        before += w < wave ? t : 0u;
+        tot += t;
+    }
+    total = tot;
+    __syncthreads();
+    return before + x - v;
+}
+
+// OR of v over the lanes of each row of 16 (DPP row rotations, which need no
+// shift direction), then over the four rows (row heads by readlane): uniform.
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+#define ATT_ROR_OR(O) v |= (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x120 + O, 0xF, 0xF, false);
+    ATT_ROR_OR(1)
+    ATT_ROR_OR(2)
+    ATT_ROR_OR(4)
+    ATT_ROR_OR(8)
+#undef ATT_ROR_OR
+    return (unsigned)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
+                      __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
+}
+
+// Lane j holds e_j, the end of the j-th slot of a chunk of 64 pairs starting
+// at B (ends strictly increasing, all above B; past the last slot, ~0u).
+// Returns, for the pair B + lane, the number of ends at or below it: the
+// ends inside (B, B + 64) are marked in one 64-bit mask m (bit e_j - B: a
+// slot starts at that lane), and the lane counts the marks at or below its
+// position.
+__device__ __forceinline__ unsigned ends_up_to(unsigned e, unsigned B, unsigned lane,
+                                               unsigned long long& m) {
+    const unsigned r = e - B;
+    const unsigned mlo = wave_or(r < 32u ? 1u << r : 0u);
+    const unsigned mhi = wave_or(r - 32u < 32u ? 1u << (r - 32u) : 0u);
+    m = ((unsigned long long)mhi << 32) | mlo;
+    const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
+    return (unsigned)__popcll(m & upto);
+}
+
+// Lane i reads lane i + O of its row of 16 (DPP row shift; O = 1, 2, 4, 8):
+// no LDS traffic, unlike a cross-lane shuffle.
+template <int O>
+__device__ __forceinline__ unsigned row_down(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x100 + O, 0xF, 0xF, false);
+}
+template <int O>
+__device__ __forceinline__ float row_down(float v) {
+    return __uint_as_float(row_down<O>(__float_as_uint(v)));
+}
+
+// Segmented sums over runs of lanes with equal `seg` (runs are contiguous:
+// seg never decreases across the wave): the first lane of each run receives
+// the sum of the run, in a fixed order.  First the suffix sums inside each
+// row of 16 lanes (DPP), then the partial of every later row whose first lane
+// continues the run, taken from those lanes in row order.
+__device__ __forceinline__ void seg_run_sums(float& Fx, float& Fy, float& Fz, unsigned seg,
+                                             unsigned lane) {
+    const unsigned col = lane & 15u;
+#define ATT_ROW_STEP(O)                                                              \
+    {                                                                                \
+        const float ox = row_down<O>(Fx), oy = row_down<O>(Fy), oz = row_down<O>(Fz); \
+        const unsigned os = row_down<O>(seg);                                        \
+        if (col + O < 16u && os == seg) { Fx += ox; Fy += oy; Fz += oz; }            \
+    }
+    ATT_ROW_STEP(1)
+    ATT_ROW_STEP(2)
+    ATT_ROW_STEP(4)
+    ATT_ROW_STEP(8)
+#undef ATT_ROW_STEP
+    const unsigned row = lane >> 4;
+#pragma unroll
+    for (unsigned r = 1; r < 4; ++r) {
+        const unsigned hs = (unsigned)__builtin_amdgcn_readlane((int)seg, 16 * r);
+        const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fx), 16 * r));
+        const float hy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fy), 16 * r));
+        const float hz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fz), 16 * r));
+        if (row < r && hs == seg) { Fx += hx; Fy += hy; Fz += hz; }
+    }
+}
+
+
+// The same with a fourth summed value (the energy of a pair).
+__device__ __forceinline__ void seg_run_sums(float& Fx, float& Fy, float& Fz, float& Fe,
+                                             unsigned seg, unsigned lane) {
+    const unsigned col = lane & 15u;
+#define ATT_ROW_STEP4(O)                                                              \
+    {                                                                                 \
+        const float ox = row_down<O>(Fx), oy = row_down<O>(Fy), oz = row_down<O>(Fz);  \
+        const float oe = row_down<O>(Fe);                                             \
+        const unsigned os = row_down<O>(seg);                                         \
+        if (col + O < 16u && os == seg) { Fx += ox; Fy += oy; Fz += oz; Fe += oe; }   \
+    }
+    ATT_ROW_STEP4(1)
+    ATT_ROW_STEP4(2)
+    ATT_ROW_STEP4(4)
+    ATT_ROW_STEP4(8)
+#undef ATT_ROW_STEP4
+    const unsigned row = lane >> 4;
+#pragma unroll
+    for (unsigned r = 1; r < 4; ++r) {
+        const unsigned hs = (unsigned)__builtin_amdgcn_readlane((int)seg, 16 * r);
+        const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fx), 16 * r));
+        const float hy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fy), 16 * r));
+        const float hz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fz), 16 * r));
+        const float he = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fe), 16 * r));
+        if (row < r && hs == seg) { Fx += hx; Fy += hy; Fz += hz; Fe += he; }
+    }
+}
+
 // Slot of workgroup b among nb in a non-persistent launch such that the
 // workgroups of one XCD (dispatched round-robin: b % 8) take one contiguous
 // range of slots: XCD x holds ceil((nb - x) / 8) workgroups, and its range
