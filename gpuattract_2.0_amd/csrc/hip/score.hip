@@ -11,9 +11,9 @@
 //  k_modes  one 256-thread workgroup per pose.  Phase A moves the deformed
 //           receptor into the ligand frame, scores it on the ligand grid and
 //           keeps its deformed coordinates in LDS; phase B scores the ligand on
-//           the receptor grid and runs the neighbour list against those LDS
-//           coordinates, folding the receptor reaction straight into the
-//           receptor mode gradients.  Semantics of
+//           the receptor grid and runs the neighbour-list pairs, one pair per
+//           lane, against those LDS coordinates, folding the receptor reaction
+//           straight into the receptor mode gradients.  Semantics of
 //           service/CPUEnergyService6DModes.tpp:121-327.
 #include <algorithm>
 
@@ -75,6 +75,18 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
 
 constexpr int kModesVals = 13 + 2 * ATT_MODES_MAX;
 
+// Beads per round of k_modes' pair-parallel neighbour-list phase.
+constexpr unsigned kModesSlots = 128;
+
+// Byte offset of that phase's slot arrays in k_modes' dynamic LDS: after the
+// pair table, the deformed receptor (xyz floats, u16 types) and the queue.
+__host__ __device__ inline size_t modes_slot_offset(const DProt& rec, const DProt& lig,
+                                                    unsigned tabSlots) {
+    const size_t q = sizeof(att_pair_params) * (size_t)tabSlots + sizeof(float) * 3 * (size_t)rec.n +
+                     sizeof(unsigned short) * (((size_t)rec.n + 1) & ~(size_t)1);
+    return (q + sizeof(unsigned) * (size_t)pow2_ceil(lig.n) + 15) & ~(size_t)15;
+}
+
 template <int MM>
 __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter,
                                                DNL nl, DGrid lInner, DGrid lOuter, DPar par,
@@ -82,7 +94,7 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
                                                float* __restrict__ res, unsigned n,
                                                unsigned tabSlots,
                                                const unsigned* __restrict__ order) {
-    extern __shared__ att_pair_params s_dyn[];
+    extern __shared__ __align__(16) att_pair_params s_dyn[];
     __shared__ float s_red[4][kModesVals];
     const att_pair_params* tab = stage_table(par, s_dyn);
     float* s_rx = reinterpret_cast<float*>(s_dyn + tabSlots);
@@ -90,6 +102,13 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     float* s_rz = s_ry + rec.n;
     unsigned short* s_rt = reinterpret_cast<unsigned short*>(s_rz + rec.n);
     unsigned* s_queue = reinterpret_cast<unsigned*>(s_rt + ((rec.n + 1) & ~1u));
+    char* dynb = reinterpret_cast<char*>(s_dyn);
+    const size_t so = modes_slot_offset(rec, lig, tabSlots);
+    float4* s_sp = reinterpret_cast<float4*>(dynb + so);                 // bead position, charge
+    unsigned* s_smeta = reinterpret_cast<unsigned*>(s_sp + kModesSlots);  // bead | type << 16
+    unsigned* s_sstart = s_smeta + kModesSlots;                           // first list entry
+    unsigned* s_send = s_sstart + kModesSlots;                            // inclusive pair prefix
+    __shared__ unsigned s_scan[4];
     __shared__ unsigned s_qn;
     if (blockIdx.x >= n) return;  // uniform per workgroup
     const unsigned pose = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
@@ -191,22 +210,110 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         if (cnt) s_queue[atomicAdd(&s_qn, 1u)] = (cnt < 0xFFFFu ? cnt : 0xFFFFu) << 16 | i;
     }
     __syncthreads();
-    // Phase B2: neighbour lists, longest first, so that the lanes of a wave
-    // walk lists of similar length.  The keys are unique, so the sorted order
-    // does not depend on the order the queue was filled in.
+    // Phase B2: the neighbour lists of the queued beads, pair-parallel.  The
+    // queue is sorted (its keys are unique, so the order does not depend on the
+    // order it was filled in); rounds of up to kModesSlots beads flatten their
+    // pairs (prefix sum of the list lengths) into one contiguous range per
+    // wave, walked in chunks of 64 pairs, one pair per lane.  Each pair is
+    // evaluated as nl_bead evaluates it (pair force minus the plateau force
+    // along the ratio-scaled direction, Coulomb likewise, the reaction into
+    // the receptor mode gradients); the terms of one bead are summed over its
+    // run of lanes (segmented sums, fixed order) and added to the pose sums by
+    // the run's first lane, once per wave that holds part of the bead (the
+    // pose sums are linear in the bead force).
     const unsigned qn = s_qn;
     const unsigned qp = pow2_ceil(qn);
     for (unsigned q = qn + threadIdx.x; q < qp; q += blockDim.x) s_queue[q] = 0;
     __syncthreads();
     bitonic_sort_desc(s_queue, qp);
-    for (unsigned q = threadIdx.x; q < qn; q += blockDim.x) {
-        const unsigned i = s_queue[q] & 0xFFFFu;
-        float4 x;
-        const V3 p = place(i, x);
-        float4 a;
-        nl_bead(nl, par, tab, recPos, recType, RecChargeG{rpos}, reaction, p.x, p.y, p.z,
-                lig.type[i], x.w, a, nullptr, nullptr);
-        add_bead(x, i, a.x, a.y, a.z, a.w);
+    const unsigned nw = blockDim.x / kWave;
+    for (unsigned base = 0; base < qn; base += kModesSlots) {
+        const unsigned ns = min(kModesSlots, qn - base);
+        unsigned cnt = 0;
+        if (threadIdx.x < ns) {
+            const unsigned i = s_queue[base + threadIdx.x] & 0xFFFFu;
+            float4 x;
+            const V3 p = place(i, x);
+            // NLGrid::getIndex as nl_bead (the bead was queued: it is in the box)
+            const int ix = (int)roundf((p.x - nl.lox) * nl.dVox_inv);
+            const int iy = (int)roundf((p.y - nl.loy) * nl.dVox_inv);
+            const int iz = (int)roundf((p.z - nl.loz) * nl.dVox_inv);
+            const uint2 dsc = nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
+            cnt = dsc.x;
+            s_sp[threadIdx.x] = make_float4(p.x, p.y, p.z, x.w);
+            s_smeta[threadIdx.x] = i | lig.type[i] << 16;
+            s_sstart[threadIdx.x] = dsc.y;
+        }
+        unsigned NP;
+        const unsigned ex = block_scan(cnt, s_scan, NP);
+        if (threadIdx.x < ns) s_send[threadIdx.x] = ex + cnt;
+        __syncthreads();
+        const unsigned per = (NP + nw - 1) / nw;
+        const unsigned rs = min(NP, (unsigned)wave * per), re = min(NP, rs + per);
+        unsigned b0 = 0;
+        {
+            unsigned lo = 0, hi = ns;  // first slot whose end exceeds rs
+            while (lo < hi) {
+                const unsigned mid = (lo + hi) >> 1;
+                if (s_send[mid] > rs) hi = mid; else lo = mid + 1;
+            }
+            b0 = lo;
+        }
+        for (unsigned B = rs; B < re; B += kWave) {
+            const unsigned pr = B + lane;
+            const bool valid = pr < re;
+            // the lane's slot: b0 + the number of slot ends at or below its pair
+            const unsigned e = b0 + lane < ns ? s_send[b0 + lane] : 0xFFFFFFFFu;
+            unsigned long long starts;
+            const unsigned sl = min(b0 + ends_up_to(e, B, lane, starts), ns - 1u);
+            b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
+            float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+            if (valid) {
+                const float4 bp = s_sp[sl];
+                const unsigned lt1 = (s_smeta[sl] >> 16) - 1u;
+                const unsigned begin = sl ? s_send[sl - 1] : 0u;
+                const unsigned nIdx = nl.list[s_sstart[sl] + (pr - begin)];
+                const float3 rp = recPos(nIdx);
+                const unsigned rt = recType(nIdx);
+                float dx = bp.x - rp.x, dy = bp.y - rp.y, dz = bp.z - rp.z;
+                const float dr2 = dx * dx + dy * dy + dz * dz;
+                // outOfPlateau (NaN skipped); a receptor bead without a valid type
+                // is skipped as in nl_bead
+                if (dr2 <= nl.P2 && rt - 1u < par.T) {
+                    const float ratio = nl.ratio[(int)(dr2 * 10000.f)];
+                    const float dr2_inv = (float)(1.0 / (double)dr2);
+                    dx *= dr2_inv; dy *= dr2_inv; dz *= dr2_inv;
+                    const att_pair_params pp = tab[par.T * (rt - 1) + lt1];
+                    float vx, vy, vz, ve;
+                    float rfx = 0.f, rfy = 0.f, rfz = 0.f;
+                    lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, vx, vy, vz, ve);
+                    fx += vx; fy += vy; fz += vz; en += ve;
+                    rfx -= vx; rfy -= vy; rfz -= vz;
+                    const float rdx = ratio * dx, rdy = ratio * dy, rdz = ratio * dz;
+                    lj(nl.P2, nl.P2_inv, rdx, rdy, rdz, pp, par.shape, vx, vy, vz, ve);
+                    fx -= vx; fy -= vy; fz -= vz; en -= ve;
+                    rfx += vx; rfy += vy; rfz += vz;
+                    if (bp.w != 0.f) {
+                        const float qlr = bp.w * rpos[nIdx].w * par.ffelec;
+                        if (fabsf(qlr) > 0.001f) {
+                            coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+                            fx += vx; fy += vy; fz += vz; en += ve;
+                            rfx -= vx; rfy -= vy; rfz -= vz;
+                            coulomb(nl.P2_inv, rdx, rdy, rdz, qlr, par.dielec, vx, vy, vz, ve);
+                            fx -= vx; fy -= vy; fz -= vz; en -= ve;
+                            rfx += vx; rfy += vy; rfz += vz;
+                        }
+                    }
+                    reaction(nIdx, rfx, rfy, rfz);
+                }
+            }
+            seg_run_sums(fx, fy, fz, en, valid ? sl : 0xFFFFFFFFu, lane);
+            if (valid && (lane == 0 || ((starts >> lane) & 1ull))) {
+                const unsigned i = s_smeta[sl] & 0xFFFFu;
+                add_bead(lig.pos[i], i, fx, fy, fz, en);
+            }
+        }
+        __syncthreads();  // slot arrays are reused by the next round
     }
 
     // Workgroup reduction: wave butterflies, then the four waves in order.
@@ -416,13 +523,12 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
 }
 
 // Dynamic LDS of k_modes: pair table, deformed receptor xyz, receptor types
-// (u16), and the neighbour-list queue (one u32 per ligand bead, power of two).
+// (u16), the neighbour-list queue (one u32 per ligand bead, power of two),
+// then (16-byte aligned) the slot arrays of the pair-parallel phase.
 size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots) {
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
-    return sizeof(att_pair_params) * slots + sizeof(float) * 3 * (size_t)rec.n +
-           sizeof(unsigned short) * (((size_t)rec.n + 1) & ~(size_t)1) +
-           sizeof(unsigned) * (size_t)pow2_ceil(lig.n);
+    return modes_slot_offset(rec, lig, slots) + kModesSlots * (16 + 3 * sizeof(unsigned));
 }
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
