@@ -108,8 +108,9 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     unsigned* s_smeta = reinterpret_cast<unsigned*>(s_sp + kModesSlots);  // bead | type << 16
     unsigned* s_sstart = s_smeta + kModesSlots;                           // first list entry
     unsigned* s_send = s_sstart + kModesSlots;                            // inclusive pair prefix
+    unsigned* s_bits = s_send + kModesSlots;  // one bit per ligand bead: has a neighbour list
+    const unsigned nWords = (lig.n + 31) / 32;
     __shared__ unsigned s_scan[4];
-    __shared__ unsigned s_qn;
     if (blockIdx.x >= n) return;  // uniform per workgroup
     const unsigned pose = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
     const int lane = threadIdx.x & (kWave - 1);
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
 #pragma unroll
     for (int m = 0; m < MA; ++m) { mR[m] = 0.f; mL[m] = 0.f; }
 
-    if (threadIdx.x == 0) s_qn = 0;
+    for (unsigned w = threadIdx.x; w < nWords; w += blockDim.x) s_bits[w] = 0u;
     // Phase A: receptor beads.
     for (unsigned j = threadIdx.x; j < rec.n; j += blockDim.x) {
         const float4 x = rec.pos[j];
@@ -200,19 +201,31 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         return p;
     };
     // Phase B1: ligand beads on the receptor grid; beads with a neighbour list
-    // are queued under the key (list length << 16 | bead).
+    // are marked.
     for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
         float4 x;
         const V3 p = place(i, x);
         const float4 pot = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
         add_bead(x, i, pot.x, pot.y, pot.z, pot.w);
         const unsigned cnt = nl_count(nl, p.x, p.y, p.z, lig.type[i]);
-        if (cnt) s_queue[atomicAdd(&s_qn, 1u)] = (cnt < 0xFFFFu ? cnt : 0xFFFFu) << 16 | i;
+        if (cnt) atomicOr(&s_bits[i >> 5], 1u << (i & 31u));
     }
     __syncthreads();
-    // Phase B2: the neighbour lists of the queued beads, pair-parallel.  The
-    // queue is sorted (its keys are unique, so the order does not depend on the
-    // order it was filled in); rounds of up to kModesSlots beads flatten their
+    // Queue: the marked beads in bead order (each thread a contiguous range of
+    // mark words; block scan of their counts).
+    unsigned qn;
+    {
+        const unsigned per = (nWords + blockDim.x - 1) / blockDim.x;
+        const unsigned lo = threadIdx.x * per, hi = min(nWords, lo + per);
+        unsigned c = 0;
+        for (unsigned w = lo; w < hi; ++w) c += (unsigned)__popc(s_bits[w]);
+        unsigned o = block_scan(c, s_scan, qn);
+        for (unsigned w = lo; w < hi; ++w)
+            for (unsigned m = s_bits[w]; m; m &= m - 1u) s_queue[o++] = 32u * w + (unsigned)__ffs(m) - 1u;
+    }
+    __syncthreads();
+    // Phase B2: the neighbour lists of the queued beads, pair-parallel: rounds
+    // of up to kModesSlots beads (in bead order) flatten their
     // pairs (prefix sum of the list lengths) into one contiguous range per
     // wave, walked in chunks of 64 pairs, one pair per lane.  Each pair is
     // evaluated as nl_bead evaluates it (pair force minus the plateau force
@@ -221,17 +234,12 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     // run of lanes (segmented sums, fixed order) and added to the pose sums by
     // the run's first lane, once per wave that holds part of the bead (the
     // pose sums are linear in the bead force).
-    const unsigned qn = s_qn;
-    const unsigned qp = pow2_ceil(qn);
-    for (unsigned q = qn + threadIdx.x; q < qp; q += blockDim.x) s_queue[q] = 0;
-    __syncthreads();
-    bitonic_sort_desc(s_queue, qp);
     const unsigned nw = blockDim.x / kWave;
     for (unsigned base = 0; base < qn; base += kModesSlots) {
         const unsigned ns = min(kModesSlots, qn - base);
         unsigned cnt = 0;
         if (threadIdx.x < ns) {
-            const unsigned i = s_queue[base + threadIdx.x] & 0xFFFFu;
+            const unsigned i = s_queue[base + threadIdx.x];
             float4 x;
             const V3 p = place(i, x);
             // NLGrid::getIndex as nl_bead (the bead was queued: it is in the box)
@@ -524,11 +532,13 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
 
 // Dynamic LDS of k_modes: pair table, deformed receptor xyz, receptor types
 // (u16), the neighbour-list queue (one u32 per ligand bead, power of two),
-// then (16-byte aligned) the slot arrays of the pair-parallel phase.
+// then (16-byte aligned) the slot arrays of the pair-parallel phase and one
+// mark bit per ligand bead.
 size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots) {
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
-    return modes_slot_offset(rec, lig, slots) + kModesSlots * (16 + 3 * sizeof(unsigned));
+    return modes_slot_offset(rec, lig, slots) + kModesSlots * (16 + 3 * sizeof(unsigned)) +
+           sizeof(unsigned) * (((size_t)lig.n + 31) / 32);
 }
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
