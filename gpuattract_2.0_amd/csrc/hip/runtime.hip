@@ -245,12 +245,14 @@ att_status check_pair(att_ctx* c, const ProtObj* rec, const ProtObj* lig, const 
 }  // namespace
 
 namespace {
-// Scoped device buffers for the diagnostic entry points.
 // Waves per pose of the single-pair adjoint and of the minimizer's scoring
-// rounds (timed best of 1, 2, 4); both use it, so a point scored by either
-// gives the same bits.
-constexpr int kPairWaves = 2;
+// rounds: one for ligands under 384 beads, two above (timed: a 300-bead
+// ligand 7.5 % faster on one wave than on two; an 800-bead ligand fastest on
+// two of 1, 2, 4).  Both paths use it, so a point scored by either gives the
+// same bits.
+int pair_waves(unsigned ligN) { return ligN < 384 ? 1 : 2; }
 
+// Scoped device buffers for the diagnostic entry points.
 struct Tmp {
     std::vector<void*> p;
     ~Tmp() {
@@ -668,7 +670,7 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
         return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
     unsigned slots = 0;
     const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, &slots);
-    constexpr int kWaves = kPairWaves;
+    const int kWaves = pair_waves(lig->d.n);
     if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
@@ -1349,6 +1351,7 @@ struct att_minjob {
     bool modes = false;
     size_t dyn = 0;
     unsigned blocks = 1;
+    int waves = 2;
     void* mem = nullptr;
     size_t bytes = 0;
     unsigned* list[2] = {nullptr, nullptr};
@@ -1398,7 +1401,8 @@ att_status att_minjob_create(att_ctx* c, att_handle hr, att_handle hl, att_handl
     std::unique_ptr<att_minjob> j(new att_minjob);
     unsigned slots = 0;
     j->dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
-    constexpr int kWaves = kPairWaves;  // the single-pair adjoint's shape: same results
+    const int kWaves = pair_waves(lig->d.n);  // the single-pair adjoint's shape: same results
+    j->waves = kWaves;
     if (j->dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     MinRun& R = j->R;
@@ -1465,7 +1469,7 @@ att_status att_minjob_rounds(att_ctx* c, att_minjob* j, int rounds, void* stream
         const int cur = j->cur, nxt = cur ^ 1;
         j->P.dofs = j->req[cur];
         j->P.nDev = j->count[cur];
-        ATT_HIP(c, launch_adjoint(j->P, j->modes, nullptr, j->blocks, kPairWaves, j->dyn, s));
+        ATT_HIP(c, launch_adjoint(j->P, j->modes, nullptr, j->blocks, j->waves, j->dyn, s));
         j->R.list = j->list[nxt]; j->R.count = j->count[nxt]; j->R.req = j->req[nxt];
         ATT_HIP(c, launch_min_step(j->R, j->list[cur], j->count[cur], j->req[cur], j->res, s));
         j->cur = nxt;

@@ -1,7 +1,8 @@
 """Batched adjoint over many uneven pairs (needs a GPU).
 
-- At the single-pair kernel's shape (2 waves per pose) a batch returns, pair by
-  pair, exactly the single-pair adjoint (same device code, same order).
+- At the single-pair kernel's shape (1 wave per pose for ligands under 384
+  beads, 2 above) a batch returns, pair by pair, exactly the single-pair
+  adjoint (same device code, same order).
 - Every launch shape (1, 2 or 4 waves per pose, any tile size) and the shape
   the autotuner picks agree with the oracle's FP32 adjoint within 1e-5 of the
   condition scale, like the single-pair adjoint.
@@ -67,8 +68,9 @@ def test_buckets(ctx, staged):
 
 def test_batch_equals_single_pair_at_same_shape(ctx, staged):
     b = _batch(ctx, staged)
-    for k in range(len(b.buckets())):
-        b.set_shape(k, 2, 16)
+    # size classes 0 and 1 hold the ligands under 384 beads
+    for k, info in enumerate(b.buckets()):
+        b.set_shape(k, 1 if info["sizeClass"] <= 1 else 2, 16)
     b.run()
     ctx.sync()
     got = _results(ctx, staged)
