@@ -1,10 +1,11 @@
 // Scoring kernels for gfx950 and their launchers.
 //
 //  k_rigid  one 64-lane wavefront per pose, four poses per 256-thread workgroup.
-//           Lanes stride over ligand beads: transform, grid interpolation,
-//           neighbour-list term, and per-lane partial sums of E, F and the
-//           torque tensor; a butterfly reduction and a one-lane epilogue
-//           (force clamp, torque matrix) write the 28-byte result.  Replaces
+//           Lanes stride over ligand beads (transform, grid interpolation),
+//           then walk the neighbour-list pairs of each chunk of 64 beads one
+//           pair per lane; per-lane partial sums of E, F and the torque
+//           tensor; a butterfly reduction and a one-lane epilogue (force
+//           clamp, torque matrix) write the 28-byte result.  Replaces
 //           d_DOF2Pos, d_potForce, d_NLPotForce, d_reduce and h_finalReduce
 //           (model/*.cu, model/reduction.h:159-207) with one launch and no
 //           intermediate buffers.
@@ -22,17 +23,32 @@
 
 namespace att {
 
+// Per-wave slot arrays of k_rigid's pair-parallel neighbour-list phase: bead
+// position and charge, bead | type << 16, first list entry, inclusive pair
+// prefix; 64 slots per wave, behind the pair table (16-byte aligned).
+constexpr unsigned kRigidSlotBytes = 64 * (16 + 3 * sizeof(unsigned));
+__host__ __device__ inline size_t rigid_slot_offset(unsigned tabSlots) {
+    return (sizeof(att_pair_params) * (size_t)tabSlots + 15) & ~(size_t)15;
+}
+
 __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer,
                                                DNL nl, DPar par, const float* __restrict__ dofs,
                                                float* __restrict__ res, unsigned n,
+                                               unsigned tabSlots,
                                                const unsigned* __restrict__ order) {
-    extern __shared__ att_pair_params s_tab[];
+    extern __shared__ __align__(16) att_pair_params s_tab[];
     const att_pair_params* tab = stage_table(par, s_tab);
     __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
+    const unsigned lane = threadIdx.x & (kWave - 1);
+    const unsigned wave = threadIdx.x / kWave;
+    char* slots = reinterpret_cast<char*>(s_tab) + rigid_slot_offset(tabSlots) + wave * kRigidSlotBytes;
+    float4* s_sp = reinterpret_cast<float4*>(slots);
+    unsigned* s_smeta = reinterpret_cast<unsigned*>(s_sp + 64);
+    unsigned* s_sstart = s_smeta + 64;
+    unsigned* s_send = s_sstart + 64;
     const unsigned wg = order ? xcd_slot(blockIdx.x, gridDim.x) : blockIdx.x;
-    const unsigned slot = wg * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (slot >= n) return;
+    const unsigned slot = wg * (blockDim.x / kWave) + wave;
+    if (slot >= n) return;  // uniform per wave; no workgroup barrier follows
     const unsigned pose = order ? order[slot] : slot;
     const float* d = dofs + 6 * (size_t)pose;
     const float tx = d[0], ty = d[1], tz = d[2];
@@ -43,22 +59,117 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
     float acc[13];
 #pragma unroll
     for (int k = 0; k < 13; ++k) acc[k] = 0.f;
+    auto add_bead = [&](const float4& x, float fx, float fy, float fz, float e) {
+        acc[0] += e;
+        acc[1] += fx; acc[2] += fy; acc[3] += fz;
+        acc[4] += x.x * fx; acc[5] += x.y * fx; acc[6] += x.z * fx;
+        acc[7] += x.x * fy; acc[8] += x.y * fy; acc[9] += x.z * fy;
+        acc[10] += x.x * fz; acc[11] += x.y * fz; acc[12] += x.z * fz;
+    };
     const float4* __restrict__ rpos = rec.pos;
-    for (unsigned i = lane; i < lig.n; i += kWave) {
-        const float4 x = lig.pos[i];
-        V3 p = matvec(R, x.x, x.y, x.z);
-        p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
-        float4 pot = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
-        float4 a;
-        if (nl_bead(nl, par, tab, RecPosG{rpos}, RecTypeG{rec.type}, RecChargeG{rpos},
-                    NoReaction{}, p.x, p.y, p.z, lig.type[i], x.w, a, nullptr, nullptr)) {
-            pot.x = pot.x + a.x; pot.y = pot.y + a.y; pot.z = pot.z + a.z; pot.w = pot.w + a.w;
+    // Chunks of 64 ligand beads, one per lane: grid term, neighbour-list cell.
+    // The beads of the chunk that have a list become the chunk's slots (in
+    // lane order); their pairs are flattened (prefix sum of the list lengths)
+    // and walked 64 at a time, one pair per lane, each evaluated as nl_bead
+    // evaluates it.  A bead's pair terms are summed over its run of lanes
+    // (segmented sums, fixed order) and added by the run's first lane (the
+    // pose sums are linear in the bead force, so the grid and the
+    // neighbour-list parts of a bead are added separately).
+    for (unsigned c0 = 0; c0 < lig.n; c0 += kWave) {
+        const unsigned i = c0 + lane;
+        unsigned cnt = 0, first = 0, lt = 0;
+        float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+        V3 p{0.f, 0.f, 0.f};
+        if (i < lig.n) {
+            x = lig.pos[i];
+            p = matvec(R, x.x, x.y, x.z);
+            p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
+            const float4 pot = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, nullptr);
+            add_bead(x, pot.x, pot.y, pot.z, pot.w);
+            lt = lig.type[i];
+            // NLGrid bounds and getIndex as nl_bead
+            if (lt != 0 && (p.x >= nl.lox && p.x <= nl.hix) && (p.y >= nl.loy && p.y <= nl.hiy) &&
+                (p.z >= nl.loz && p.z <= nl.hiz)) {
+                const int ix = (int)roundf((p.x - nl.lox) * nl.dVox_inv);
+                const int iy = (int)roundf((p.y - nl.loy) * nl.dVox_inv);
+                const int iz = (int)roundf((p.z - nl.loz) * nl.dVox_inv);
+                const uint2 dsc = nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
+                cnt = dsc.x;
+                first = dsc.y;
+            }
         }
-        acc[0] += pot.w;
-        acc[1] += pot.x; acc[2] += pot.y; acc[3] += pot.z;
-        acc[4] += x.x * pot.x; acc[5] += x.y * pot.x; acc[6] += x.z * pot.x;
-        acc[7] += x.x * pot.y; acc[8] += x.y * pot.y; acc[9] += x.z * pot.y;
-        acc[10] += x.x * pot.z; acc[11] += x.y * pot.z; acc[12] += x.z * pot.z;
+        const unsigned long long hasList = __ballot(cnt != 0u);
+        const unsigned ns = (unsigned)__popcll(hasList);
+        if (ns == 0) continue;
+        // inclusive prefix of the list lengths over the lanes
+        unsigned incl = cnt;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const unsigned y = __shfl_up(incl, o);
+            if (lane >= (unsigned)o) incl += y;
+        }
+        const unsigned NP = __shfl(incl, kWave - 1);
+        if (cnt) {
+            const unsigned k = (unsigned)__popcll(hasList & ((1ull << lane) - 1ull));
+            s_sp[k] = make_float4(p.x, p.y, p.z, x.w);
+            s_smeta[k] = i | lt << 16;
+            s_sstart[k] = first;
+            s_send[k] = incl;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        unsigned b0 = 0;
+        for (unsigned B = 0; B < NP; B += kWave) {
+            const unsigned pr = B + lane;
+            const bool valid = pr < NP;
+            // the lane's slot: b0 + the number of slot ends at or below its pair
+            const unsigned e = b0 + lane < ns ? s_send[b0 + lane] : 0xFFFFFFFFu;
+            unsigned long long starts;
+            const unsigned sl = min(b0 + ends_up_to(e, B, lane, starts), ns - 1u);
+            b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
+            float fx = 0.f, fy = 0.f, fz = 0.f, en = 0.f;
+            if (valid) {
+                const float4 bp = s_sp[sl];
+                const unsigned lt1 = (s_smeta[sl] >> 16) - 1u;
+                const unsigned begin = sl ? s_send[sl - 1] : 0u;
+                const unsigned nIdx = nl.list[s_sstart[sl] + (pr - begin)];
+                const float4 rx = rpos[nIdx];
+                const unsigned rt = rec.type[nIdx];
+                float dx = bp.x - rx.x, dy = bp.y - rx.y, dz = bp.z - rx.z;
+                const float dr2 = dx * dx + dy * dy + dz * dz;
+                // outOfPlateau (NaN skipped); a receptor bead without a valid type
+                // is skipped as in nl_bead
+                if (dr2 <= nl.P2 && rt - 1u < par.T) {
+                    const float ratio = nl.ratio[(int)(dr2 * 10000.f)];
+                    const float dr2_inv = (float)(1.0 / (double)dr2);
+                    dx *= dr2_inv; dy *= dr2_inv; dz *= dr2_inv;
+                    const att_pair_params pp = tab[par.T * (rt - 1) + lt1];
+                    float vx, vy, vz, ve;
+                    lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, vx, vy, vz, ve);
+                    fx += vx; fy += vy; fz += vz; en += ve;
+                    const float rdx = ratio * dx, rdy = ratio * dy, rdz = ratio * dz;
+                    lj(nl.P2, nl.P2_inv, rdx, rdy, rdz, pp, par.shape, vx, vy, vz, ve);
+                    fx -= vx; fy -= vy; fz -= vz; en -= ve;
+                    if (bp.w != 0.f) {
+                        const float qlr = bp.w * rx.w * par.ffelec;
+                        if (fabsf(qlr) > 0.001f) {
+                            coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+                            fx += vx; fy += vy; fz += vz; en += ve;
+                            coulomb(nl.P2_inv, rdx, rdy, rdz, qlr, par.dielec, vx, vy, vz, ve);
+                            fx -= vx; fy -= vy; fz -= vz; en -= ve;
+                        }
+                    }
+                }
+            }
+            seg_run_sums(fx, fy, fz, en, valid ? sl : 0xFFFFFFFFu, lane);
+            if (valid && (lane == 0 || ((starts >> lane) & 1ull)))
+                add_bead(lig.pos[s_smeta[sl] & 0xFFFFu], fx, fy, fz, en);
+        }
+        // every lane has read the slots before the next chunk writes them
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 #pragma unroll
     for (int k = 0; k < 13; ++k) acc[k] = wave_sum(acc[k]);
@@ -524,9 +635,10 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
     if (n == 0) return hipSuccess;
     const unsigned posesPerBlock = 4;
     const unsigned blocks = (n + posesPerBlock - 1) / posesPerBlock;
-    const size_t lds = par.T <= kMaxTypesLDS ? sizeof(att_pair_params) * par.T * par.T : 0;
+    const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
+    const size_t lds = rigid_slot_offset(slots) + posesPerBlock * kRigidSlotBytes;
     hipLaunchKernelGGL(k_rigid, dim3(blocks), dim3(256), lds, s, rec, lig, inner, outer, nl, par,
-                       dofs, res, n, order);
+                       dofs, res, n, slots, order);
     return hipGetLastError();
 }
 

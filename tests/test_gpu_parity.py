@@ -190,7 +190,11 @@ def test_edge_cases(ctx, W, O):
     hs = ctx.upload_workload(wl)
     g = ctx.score_rigid(hs, wl.dofs)
     o64, sc = O.score_rigid(wl, f64=True)
-    assert np.array_equal(g, O.score_rigid(wl))  # one bead: no summation-order freedom
+    # one bead: the only summation-order freedom left is its neighbour-list
+    # pairs, which the scoring kernel sums as a tree (one pair per lane) and
+    # the oracle in list order; the per-bead values are bit-exact through the
+    # diagnostics path (_check_beads)
+    assert cond_err(g, O.score_rigid(wl), sc).max() <= TOL
     assert cond_err(g, o64, sc).max() <= 1e-3
     # poses far outside every grid score exactly zero
     far = wl.dofs.copy()
