@@ -209,6 +209,14 @@ def main():
 
     # The plain per-item path: host buffers, 100 poses per call (the reference's
     # item size, server/Server.h:39), as the drop-in service issues them.
+    # the same adjoint from host buffers in one call (DOFs over PCIe in, results
+    # and parameter gradients out): the PCIe-inclusive rate, never `value`
+    ctx.adjoint(hs, wl.dofs[:1000], pgrad=pgrad, T=T)
+    t0 = time.perf_counter()
+    ctx.adjoint(hs, wl.dofs, pgrad=pgrad, T=T)
+    host_adj = n / (time.perf_counter() - t0)
+    log(f"adjoint from host buffers, one call: {host_adj:.4g} poses/s")
+
     m = min(args.plain_poses, n)
     host_score = ctx.score_modes if wl.modes else ctx.score_rigid
     host_score(hs, wl.dofs[:100])
@@ -261,6 +269,7 @@ def main():
             "adjoint_dof_modes_only_poses_per_s": n / dof_ms * 1e3,
             "adjoint_input_order_poses_per_s": n / unordered_ms * 1e3,
             "forward_kernel_poses_per_s": n / fwd_ms * 1e3,
+            "adjoint_host_buffers_poses_per_s": host_adj,
             "plain_item100_poses_per_s": plain,
             "adjoint_vs_plain": value / plain,
             "server_item100_poses_per_s": server,
