@@ -219,3 +219,26 @@ def test_edge_cases(ctx, W, O):
     with pytest.raises(hip.AttError) as e:
         ctx.score_rigid(dict(hs, params=hp), wl.dofs)
     assert e.value.status == 1
+
+
+def test_many_types_table_in_global_memory(ctx, W, O):
+    """40 bead types: the pair table (1600 entries) exceeds what the kernels
+    stage in LDS (32x32), so they read it from global memory and their LDS
+    slot arrays start at offset 0; forward and adjoint still match the oracle."""
+    wl = W.make_workload("many types", 31, 300, 90, 64, num_types=40)
+    hs = ctx.upload_workload(wl)
+    g = ctx.score_rigid(hs, wl.dofs)
+    o64, sc = O.score_rigid(wl, f64=True)
+    ok = forcelim_safe(o64)
+    assert cond_err(g[ok], O.score_rigid(wl)[ok], sc[ok]).max() <= TOL
+    a = ctx.adjoint(hs, wl.dofs)
+    r32 = O.adjoint(wl, f32=True)
+    _, sca = O.adjoint(wl, scale=True)
+    cols = [c for c in range(7) if sca[:, c].max() > 0]
+    assert cond_err(a[:, cols], r32[:, cols], sca[:, cols]).max() <= TOL
+    wm = W.make_workload("many types modes", 32, 300, 90, 32, 2, 3, num_types=40)
+    hm = ctx.upload_workload(wm)
+    gm = ctx.score_modes(hm, wm.dofs)
+    _, scm = O.score_modes(wm, f64=True)
+    colsm = scm.max(axis=0) > 0
+    assert cond_err(gm[:, colsm], O.score_modes(wm)[:, colsm], scm[:, colsm]).max() <= TOL
