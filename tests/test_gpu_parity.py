@@ -242,3 +242,22 @@ def test_many_types_table_in_global_memory(ctx, W, O):
     _, scm = O.score_modes(wm, f64=True)
     colsm = scm.max(axis=0) > 0
     assert cond_err(gm[:, colsm], O.score_modes(wm)[:, colsm], scm[:, colsm]).max() <= TOL
+
+
+@pytest.mark.parametrize("n_lig", [63, 64, 65, 129])
+def test_bead_chunk_boundaries(ctx, W, O, n_lig):
+    """Ligands around the 64-bead chunks of the rigid kernel's pair-parallel
+    phase and the 128-bead rounds of the modes kernel: forward results match
+    the oracle."""
+    wl = W.make_workload(f"chunks {n_lig}", 40 + n_lig, 250, n_lig, 48)
+    hs = ctx.upload_workload(wl)
+    g = ctx.score_rigid(hs, wl.dofs)
+    o64, sc = O.score_rigid(wl, f64=True)
+    ok = forcelim_safe(o64)
+    assert cond_err(g[ok], O.score_rigid(wl)[ok], sc[ok]).max() <= TOL
+    wm = W.make_workload(f"chunks modes {n_lig}", 50 + n_lig, 250, n_lig, 24, 2, 2)
+    hm = ctx.upload_workload(wm)
+    gm = ctx.score_modes(hm, wm.dofs)
+    _, scm = O.score_modes(wm, f64=True)
+    cols = scm.max(axis=0) > 0
+    assert cond_err(gm[:, cols], O.score_modes(wm)[:, cols], scm[:, cols]).max() <= TOL
