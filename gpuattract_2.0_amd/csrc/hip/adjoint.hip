@@ -195,22 +195,20 @@ struct PairLDS {
     float red[WAVES][13 + 2 * ATT_MODES_MAX];
 };
 
-// Adjoint of one pose by one workgroup of WAVES waves, in chunks of kSlots
-// ligand beads (one per thread):
-//  B1   transform, grid term with its exact gradient (and the basis-grid
-//       parameter derivatives), neighbour-list cell; the chunk's beads with a
-//       list become its slots, in bead order (block scans of the flags and of
-//       the list lengths).
-//  B2   the slots' pairs are flattened (prefix sum of the list lengths) and
-//       split into WAVES contiguous ranges, one per wave, walked in chunks of
-//       64 consecutive pairs: list reads are coalesced and every lane has work
-//       until the range ends.  Receptor beads are deformed on the fly from
-//       their padded mode row (one cache line).  Pair forces are summed per
-//       bead by segmented wave sums (DPP); a bead split between two waves gets
-//       the later wave's share through `head`, added in wave order, so results
-//       are deterministic.
-//  C    each bead's thread applies the chain rule once, to its grid gradient
-//       minus its neighbour-list force.
+// Adjoint of one pose by one workgroup of WAVES waves.
+//  B1   every ligand bead: transform, grid term with its exact gradient (and
+//       the basis-grid parameter derivatives), chain rule; beads with a
+//       neighbour list are recorded with their {count, start}.
+//  Q    the recorded beads are compacted in bead order (block scan).
+//  B2   pair rounds of up to kSlots beads: the pairs of the round are
+//       flattened (prefix sum of counts) and split into WAVES contiguous
+//       ranges, one per wave, walked in chunks of 64 consecutive pairs: list
+//       reads are coalesced and every lane has work until the range ends.
+//       Receptor beads are deformed on the fly from their padded mode row
+//       (one cache line).  Pair forces are summed per bead by a segmented
+//       wave reduction; a bead split between two waves gets the later wave's
+//       share through `head`, added in wave order, so results are
+//       deterministic.  Then the chain rule for the NL force of each bead.
 template <int MM, int WAVES>
 __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, PairLDS<WAVES>& S,
                                              char* dyn, const att_pair_params* tab, float* s_pg) {
@@ -218,6 +216,8 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
     const DProt& lig = P.lig;
     const DNL& nl = P.nl;
     const DPar& par = P.par;
+    uint2* s_nl = reinterpret_cast<uint2*>(dyn + adjoint_nl_offset(P.tabSlots));
+    unsigned short* s_queue = reinterpret_cast<unsigned short*>(dyn + adjoint_queue_offset(P.tabSlots, lig.n));
     constexpr unsigned kSlots = PairLDS<WAVES>::kSlots;
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned wave = threadIdx.x / kWave;
@@ -276,176 +276,189 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
             }
         };
 
-        // Chunks of kSlots ligand beads, one per thread: the grid term with its
-        // exact gradient (and the basis-grid parameter derivatives) and the
-        // bead's neighbour-list cell; the chunk's beads with a list become its
-        // slots, in bead order, and their pairs are scored (below); then each
-        // bead's own thread applies the chain rule once, to its grid gradient
-        // minus its neighbour-list force.
-        for (unsigned c0 = 0; c0 < lig.n; c0 += kSlots) {
-            const unsigned bi = c0 + threadIdx.x;
-            float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
-            V3 bxt{0.f, 0.f, 0.f};
-            float3 bG = make_float3(0.f, 0.f, 0.f);
-            uint2 cell = make_uint2(0u, 0u);
-            V3 bpos{0.f, 0.f, 0.f};
-            if (bi < lig.n) {
-                bpos = place(bi, bx, bxt);
-                acc[0] += grid_adj(P.inner, P.outer, P.bin, P.bout, par, bpos.x, bpos.y, bpos.z,
-                                   lig.mtype[bi], bx.w, bG, pg);
-                cell = nl_cell(nl, bpos.x, bpos.y, bpos.z, lig.type[bi]);
+        // B1: grid term of every ligand bead; neighbour-list cells recorded.
+        for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
+            float4 x;
+            V3 xt;
+            const V3 p = place(i, x, xt);
+            float3 G;
+            acc[0] += grid_adj(P.inner, P.outer, P.bin, P.bout, par, p.x, p.y, p.z, lig.mtype[i], x.w, G, pg);
+            add_grad(i, xt.x, xt.y, xt.z, G.x, G.y, G.z);
+            s_nl[i] = nl_cell(nl, p.x, p.y, p.z, lig.type[i]);
+        }
+        __syncthreads();
+
+        // Q: beads with listed neighbours, in bead order.
+        {
+            const unsigned per = (lig.n + blockDim.x - 1) / blockDim.x;
+            const unsigned lo = threadIdx.x * per, hi = min(lig.n, lo + per);
+            unsigned cnt = 0;
+            for (unsigned i = lo; i < hi; ++i) cnt += s_nl[i].x != 0u;
+            unsigned total;
+            unsigned o = block_scan(cnt, S.scan, total);
+            for (unsigned i = lo; i < hi; ++i)
+                if (s_nl[i].x != 0u) s_queue[o++] = (unsigned short)i;
+            if (threadIdx.x == 0) S.qn = total;
+        }
+        __syncthreads();
+        const unsigned qn = S.qn;
+
+        // B2: pair rounds.
+        for (unsigned base = 0; base < qn; base += kSlots) {
+            const unsigned ns = min(kSlots, qn - base);
+            unsigned cnt = 0;
+            if (threadIdx.x < ns) {
+                const unsigned i = s_queue[base + threadIdx.x];
+                float4 x;
+                V3 xt;
+                const V3 p = place(i, x, xt);
+                const uint2 c = s_nl[i];
+                cnt = c.x;
+                S.p[threadIdx.x] = make_float4(p.x, p.y, p.z, x.w);
+                S.x[threadIdx.x] = make_float4(xt.x, xt.y, xt.z, __uint_as_float(i | lig.type[i] << 16));
+                S.start[threadIdx.x] = c.y;
+                S.G[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            unsigned ns;
-            const unsigned k = block_scan(cell.x != 0u ? 1u : 0u, S.scan, ns);
             unsigned NP;
-            const unsigned ex = block_scan(cell.x, S.scan, NP);
-            if (cell.x != 0u) {
-                S.p[k] = make_float4(bpos.x, bpos.y, bpos.z, bx.w);
-                S.x[k] = make_float4(bxt.x, bxt.y, bxt.z, __uint_as_float(bi | lig.type[bi] << 16));
-                S.start[k] = cell.y;
-                S.end[k] = ex + cell.x;
-                S.G[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+            const unsigned ex = block_scan(cnt, S.scan, NP);
+            if (threadIdx.x < ns) S.end[threadIdx.x] = ex + cnt;
             if (lane == 0) S.head[wave] = make_float4(0.f, 0.f, 0.f, 0.f);
             __syncthreads();
-            if (NP) {
-                // this wave's range of pairs and the slot it starts in
-                const unsigned per = (NP + WAVES - 1) / WAVES;
-                const unsigned rs = min(NP, wave * per), re = min(NP, rs + per);
-                unsigned b0 = 0;
-                {
-                    unsigned lo = 0, hi = ns;  // first slot whose end exceeds rs
+
+            // this wave's range of pairs and the slot it starts in
+            const unsigned per = (NP + WAVES - 1) / WAVES;
+            const unsigned rs = min(NP, wave * per), re = min(NP, rs + per);
+            unsigned b0 = 0;
+            {
+                unsigned lo = 0, hi = ns;  // first slot whose end exceeds rs
+                while (lo < hi) {
+                    const unsigned mid = (lo + hi) >> 1;
+                    if (S.end[mid] > rs) hi = mid; else lo = mid + 1;
+                }
+                b0 = lo;
+            }
+            const unsigned firstSlot = b0;
+            const bool headSplit = rs < re && rs > (b0 ? S.end[b0 - 1] : 0u);
+            for (unsigned B = rs; B < re; B += kWave) {
+                const unsigned p = B + lane;
+                const bool valid = p < re;
+                // Slot of every lane: lane j holds the end of slot b0 + j (ends
+                // increase strictly, the first exceeds B); the lane's slot is b0 +
+                // the number of those ends at or below its pair (ends_up_to).
+                const unsigned e = b0 + lane < ns ? S.end[b0 + lane] : 0xFFFFFFFFu;
+                unsigned long long starts;
+                const unsigned s = min(b0 + ends_up_to(e, B, lane, starts), ns - 1u);
+                b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
+
+                float Fx = 0.f, Fy = 0.f, Fz = 0.f;
+                if (valid) {
+                    const float4 bp = S.p[s];
+                    const unsigned meta = __float_as_uint(S.x[s].w);
+                    const unsigned lt1 = (meta >> 16) - 1u;
+                    const unsigned begin = s ? S.end[s - 1] : 0u;
+                    const unsigned j = nl.list[S.start[s] + (p - begin)];
+                    const float4 rx = rpos[j];
+                    const unsigned rt = rec.type[j];
+                    float mt[MM > 0 ? 4 * ((3 * MA + 3) / 4) : 4];
+                    float yx = rx.x, yy = rx.y, yz = rx.z;
+                    if (MM > 0) {
+                        constexpr unsigned Q = (3 * MA + 3) / 4;
+                        const float4* row = rmodesT + (size_t)j * rowT;
+#pragma unroll
+                        for (unsigned q = 0; q < Q; ++q) {
+                            const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                            mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
+                        }
+                        // the deformation of deform(), same order of operations
+#pragma unroll
+                        for (unsigned m = 0; m < (unsigned)MA; ++m)
+                            if (m < Mr) {
+                                const float a = ampR[m];
+                                yx = yx + a * mt[3 * m];
+                                yy = yy + a * mt[3 * m + 1];
+                                yz = yz + a * mt[3 * m + 2];
+                            }
+                    }
+                    const float ddx = bp.x - yx, ddy = bp.y - yy, ddz = bp.z - yz;
+                    const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    if (dr2 <= nl.P2 && rt - 1u < par.T) {
+                        const float dr2_inv = (float)(1.0 / (double)dr2);
+                        const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
+                        const unsigned kk = par.T * (rt - 1) + lt1;
+                        const att_pair_params pp = tab[kk];
+                        float vx, vy, vz, ve;
+                        lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, Fx, Fy, Fz, ve);
+                        float e2 = ve;
+                        lj(nl.P2, nl.P2_inv, 0.f, 0.f, 0.f, pp, par.shape, vx, vy, vz, ve);
+                        e2 -= ve;
+                        if (bp.w != 0.f) {
+                            const float qlr = bp.w * rx.w * par.ffelec;
+                            if (fabsf(qlr) > 0.001f) {
+                                coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+                                e2 += ve;
+                                Fx += vx; Fy += vy; Fz += vz;
+                                coulomb(nl.P2_inv, 0.f, 0.f, 0.f, qlr, par.dielec, vx, vy, vz, ve);
+                                e2 -= ve;
+                            }
+                        }
+                        acc[0] += e2;
+                        if (MM > 0) {
+#pragma unroll
+                            for (unsigned m = 0; m < (unsigned)MA; ++m)
+                                if (m < Mr) gR[m] += Fx * mt[3 * m] + Fy * mt[3 * m + 1] + Fz * mt[3 * m + 2];
+                        }
+                        if (pg.s) {
+                            float drc, dac, dem, drcP, dacP, demP;
+                            lj_dparam(dr2, dr2_inv, pp, par.shape, drc, dac, dem);
+                            lj_dparam(nl.P2, nl.P2_inv, pp, par.shape, drcP, dacP, demP);
+                            atomicAdd(&pg.s[3 * kk], pg.w * (drc - drcP));
+                            atomicAdd(&pg.s[3 * kk + 1], pg.w * (dac - dacP));
+                            if (dem != demP) atomicAdd(&pg.s[3 * kk + 2], pg.w * (dem - demP));
+                        }
+                    }
+                }
+                // sum of the forces of each run of lanes sharing a slot, in its first lane
+                const unsigned seg = valid ? s : 0xFFFFFFFFu;
+                seg_run_sums(Fx, Fy, Fz, seg, lane);
+                // the first lane of a run: lane 0, or a slot starts at the lane
+                if (valid && (lane == 0 || ((starts >> lane) & 1ull))) {
+                    if (headSplit && seg == firstSlot) {
+                        float4 h = S.head[wave];
+                        h.x += Fx; h.y += Fy; h.z += Fz;
+                        S.head[wave] = h;
+                    } else {
+                        float4 g = S.G[seg];
+                        g.x += Fx; g.y += Fy; g.z += Fz;
+                        S.G[seg] = g;
+                    }
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                // shares of beads split between waves, in wave order
+                for (unsigned w = 1; w < (unsigned)WAVES; ++w) {
+                    const unsigned ws = min(NP, w * per);
+                    if (ws >= NP) break;
+                    unsigned lo = 0, hi = ns;
                     while (lo < hi) {
                         const unsigned mid = (lo + hi) >> 1;
-                        if (S.end[mid] > rs) hi = mid; else lo = mid + 1;
+                        if (S.end[mid] > ws) hi = mid; else lo = mid + 1;
                     }
-                    b0 = lo;
-                }
-                const unsigned firstSlot = b0;
-                const bool headSplit = rs < re && rs > (b0 ? S.end[b0 - 1] : 0u);
-                for (unsigned B = rs; B < re; B += kWave) {
-                    const unsigned p = B + lane;
-                    const bool valid = p < re;
-                    // Slot of every lane: lane j holds the end of slot b0 + j (ends
-                    // increase strictly, the first exceeds B); the lane's slot is b0 +
-                    // the number of those ends at or below its pair (ends_up_to).
-                    const unsigned e = b0 + lane < ns ? S.end[b0 + lane] : 0xFFFFFFFFu;
-                    unsigned long long starts;
-                    const unsigned s = min(b0 + ends_up_to(e, B, lane, starts), ns - 1u);
-                    b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
-
-                    float Fx = 0.f, Fy = 0.f, Fz = 0.f;
-                    if (valid) {
-                        const float4 bp = S.p[s];
-                        const unsigned meta = __float_as_uint(S.x[s].w);
-                        const unsigned lt1 = (meta >> 16) - 1u;
-                        const unsigned begin = s ? S.end[s - 1] : 0u;
-                        const unsigned j = nl.list[S.start[s] + (p - begin)];
-                        const float4 rx = rpos[j];
-                        const unsigned rt = rec.type[j];
-                        float mt[MM > 0 ? 4 * ((3 * MA + 3) / 4) : 4];
-                        float yx = rx.x, yy = rx.y, yz = rx.z;
-                        if (MM > 0) {
-                            constexpr unsigned Q = (3 * MA + 3) / 4;
-                            const float4* row = rmodesT + (size_t)j * rowT;
-    #pragma unroll
-                            for (unsigned q = 0; q < Q; ++q) {
-                                const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                                mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
-                            }
-                            // the deformation of deform(), same order of operations
-    #pragma unroll
-                            for (unsigned m = 0; m < (unsigned)MA; ++m)
-                                if (m < Mr) {
-                                    const float a = ampR[m];
-                                    yx = yx + a * mt[3 * m];
-                                    yy = yy + a * mt[3 * m + 1];
-                                    yz = yz + a * mt[3 * m + 2];
-                                }
-                        }
-                        const float ddx = bp.x - yx, ddy = bp.y - yy, ddz = bp.z - yz;
-                        const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        if (dr2 <= nl.P2 && rt - 1u < par.T) {
-                            const float dr2_inv = (float)(1.0 / (double)dr2);
-                            const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
-                            const unsigned kk = par.T * (rt - 1) + lt1;
-                            const att_pair_params pp = tab[kk];
-                            float vx, vy, vz, ve;
-                            lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, Fx, Fy, Fz, ve);
-                            float e2 = ve;
-                            lj(nl.P2, nl.P2_inv, 0.f, 0.f, 0.f, pp, par.shape, vx, vy, vz, ve);
-                            e2 -= ve;
-                            if (bp.w != 0.f) {
-                                const float qlr = bp.w * rx.w * par.ffelec;
-                                if (fabsf(qlr) > 0.001f) {
-                                    coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
-                                    e2 += ve;
-                                    Fx += vx; Fy += vy; Fz += vz;
-                                    coulomb(nl.P2_inv, 0.f, 0.f, 0.f, qlr, par.dielec, vx, vy, vz, ve);
-                                    e2 -= ve;
-                                }
-                            }
-                            acc[0] += e2;
-                            if (MM > 0) {
-    #pragma unroll
-                                for (unsigned m = 0; m < (unsigned)MA; ++m)
-                                    if (m < Mr) gR[m] += Fx * mt[3 * m] + Fy * mt[3 * m + 1] + Fz * mt[3 * m + 2];
-                            }
-                            if (pg.s) {
-                                float drc, dac, dem, drcP, dacP, demP;
-                                lj_dparam(dr2, dr2_inv, pp, par.shape, drc, dac, dem);
-                                lj_dparam(nl.P2, nl.P2_inv, pp, par.shape, drcP, dacP, demP);
-                                atomicAdd(&pg.s[3 * kk], pg.w * (drc - drcP));
-                                atomicAdd(&pg.s[3 * kk + 1], pg.w * (dac - dacP));
-                                if (dem != demP) atomicAdd(&pg.s[3 * kk + 2], pg.w * (dem - demP));
-                            }
-                        }
-                    }
-                    // sum of the forces of each run of lanes sharing a slot, in its first lane
-                    const unsigned seg = valid ? s : 0xFFFFFFFFu;
-                    seg_run_sums(Fx, Fy, Fz, seg, lane);
-                    // the first lane of a run: lane 0, or a slot starts at the lane
-                    if (valid && (lane == 0 || ((starts >> lane) & 1ull))) {
-                        if (headSplit && seg == firstSlot) {
-                            float4 h = S.head[wave];
-                            h.x += Fx; h.y += Fy; h.z += Fz;
-                            S.head[wave] = h;
-                        } else {
-                            float4 g = S.G[seg];
-                            g.x += Fx; g.y += Fy; g.z += Fz;
-                            S.G[seg] = g;
-                        }
+                    if (ws > (lo ? S.end[lo - 1] : 0u)) {
+                        float4 g = S.G[lo];
+                        const float4 h = S.head[w];
+                        g.x += h.x; g.y += h.y; g.z += h.z;
+                        S.G[lo] = g;
                     }
                 }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    // shares of beads split between waves, in wave order
-                    for (unsigned w = 1; w < (unsigned)WAVES; ++w) {
-                        const unsigned ws = min(NP, w * per);
-                        if (ws >= NP) break;
-                        unsigned lo = 0, hi = ns;
-                        while (lo < hi) {
-                            const unsigned mid = (lo + hi) >> 1;
-                            if (S.end[mid] > ws) hi = mid; else lo = mid + 1;
-                        }
-                        if (ws > (lo ? S.end[lo - 1] : 0u)) {
-                            float4 g = S.G[lo];
-                            const float4 h = S.head[w];
-                            g.x += h.x; g.y += h.y; g.z += h.z;
-                            S.G[lo] = g;
-                        }
-                    }
-                }
-                __syncthreads();
             }
-            if (bi < lig.n) {
-                if (cell.x != 0u) {
-                    const float4 g = S.G[k];
-                    bG.x -= g.x; bG.y -= g.y; bG.z -= g.z;
-                }
-                add_grad(bi, bxt.x, bxt.y, bxt.z, bG.x, bG.y, bG.z);
+            __syncthreads();
+            // chain rule for the neighbour-list force of each bead of the round
+            if (threadIdx.x < ns) {
+                const float4 g = S.G[threadIdx.x], xs = S.x[threadIdx.x];
+                add_grad(__float_as_uint(xs.w) & 0xFFFFu, xs.x, xs.y, xs.z, -g.x, -g.y, -g.z);
             }
-            __syncthreads();  // slot arrays are reused by the next chunk
+            __syncthreads();  // slot arrays are reused by the next round
         }
 
         // Reduction and epilogue.
