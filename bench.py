@@ -186,26 +186,12 @@ def main():
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {wall:.3f} s, {value:.4g} poses/s, launch avg {avg_launch*1e3:.3f} ms")
 
-    if args.copy_calibration:
-        nb = args.copy_calibration // 16 * 16
-        a, bb = ctx.dev_alloc(nb), ctx.dev_alloc(nb)
-        ctx.copy(bb, a, nb, sh)
-        ctx.sync(sh)
-        log(f"copy calibration: {nb} bytes read + {nb} bytes written")
-        a.release()
-        bb.release()
-
+    copy_calibration(args, ctx, sh)
     bpp, bpp_sd = traffic.bytes_per_pose(wl, adjoint=True, pgrad=pgrad)
     achieved = bpp * n / avg_launch
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None,
+                    frac=achieved / HBM_PEAK, traffic=profiled_traffic(args.workload) if pgrad else None,
                     bytes_per_pose=bpp, kernel="k_adjoint")
-    prof = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(prof):
-        try:
-            roofline["traffic"] = json.load(open(prof)).get("hbm_bytes_per_launch")
-        except Exception:
-            pass
 
     # The plain per-item path: host buffers, 100 poses per call (the reference's
     # item size, server/Server.h:39), as the drop-in service issues them.
@@ -510,6 +496,30 @@ def server_rate_batched(np, hip, W, ctx, staged, total):
     return rate
 
 
+def copy_calibration(args, ctx, sh):
+    """The plain copy kernel over --copy-calibration bytes (read once, written
+    once), so that a counter run can scale FETCH_SIZE / WRITE_SIZE."""
+    if not args.copy_calibration:
+        return
+    nb = args.copy_calibration // 16 * 16
+    a, bb = ctx.dev_alloc(nb), ctx.dev_alloc(nb)
+    ctx.copy(bb, a, nb, sh)
+    ctx.sync(sh)
+    log(f"copy calibration: {nb} bytes read + {nb} bytes written")
+    a.release()
+    bb.release()
+
+
+def profiled_traffic(workload):
+    """HBM bytes per launch (per graph replay for the batched workload) from
+    the calibrated FETCH_SIZE / WRITE_SIZE counter runs kept in profiles/."""
+    prof = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    try:
+        return json.load(open(prof)).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
 def run_batched(args, np, hip, traffic, W):
     """Uneven pairs (receptor 500-5000 beads, ligand 100-1500, 0-10 modes per
     body, 1000-50000 poses, all log-uniform except modes) scored by the batched
@@ -552,6 +562,7 @@ def run_batched(args, np, hip, traffic, W):
     launch = float(np.mean([ctx.elapsed_ms(a, c) for a, c in zip(starts, ends)])) / 1e3
     value = total * args.steps / wall
     log(f"timed {args.steps} graph replays: {value:.4g} poses/s, {launch * 1e3:.3f} ms per replay")
+    copy_calibration(args, ctx, sh)
 
     # the same work pair by pair: one device adjoint launch per pair
     e0, e1 = ctx.event(), ctx.event()
@@ -576,7 +587,7 @@ def run_batched(args, np, hip, traffic, W):
     bpp = [traffic.bytes_per_pose(s["wl"], sample=64, adjoint=True)[0] for s in staged]
     algo = sum(bp * s["n"] for bp, s in zip(bpp, staged))
     roofline = dict(bound="hbm", achieved=algo / launch / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=algo / launch / HBM_PEAK, traffic=None,
+                    frac=algo / launch / HBM_PEAK, traffic=profiled_traffic("batched"),
                     bytes_per_pose=algo / total, kernel="k_adjoint_batch")
     cpu = None
     if not args.no_cpu:
