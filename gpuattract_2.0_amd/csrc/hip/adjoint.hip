@@ -500,8 +500,17 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
     }
 }
 
+// Occupancy per mode capacity: the 10-mode kernels run at 3 waves per SIMD
+// (168 VGPRs).  At 4 their per-lane mode amplitudes and gradients spill twice
+// the scratch (252-332 against 84-168 B per lane); at 3 a 4000/1000-bead
+// pair with 10 + 10 modes scores 10 % faster and the 64-pair batched replay
+// takes 2.3 % less time.  The 0- and 5-mode kernels stay at 4 (3 measured
+// slower).  Occupancy changes no arithmetic: results are the same bits.
+template <int MM>
+constexpr int adjoint_waves_per_simd() { return MM >= 10 ? 3 : 4; }
+
 template <int MM, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint(
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<MM>()))) void k_adjoint(
     PairDesc P, float* __restrict__ pgPartial) {
     extern __shared__ att_pair_params s_dyn[];
     __shared__ PairLDS<WAVES> S;
@@ -537,12 +546,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(4)))
 // drains queue b % 8, then helps with the others.  Queue x holds the x-th
 // eighth of every pair's poses in that pair's spatial order, so one XCD's L2
 // sees neighbouring poses.  The pair's descriptor is read with scalar loads
-// and its table staged in LDS when the pair changes.  The 10-mode kernels run
-// at 3 waves per SIMD (168 VGPRs): at 4 their mode amplitudes and gradients
-// spill twice the scratch (324 against 160 B per lane for two waves per pose)
-// and the 64-pair batched replay takes 2.3 % longer; the others stay at 4.
+// and its table staged in LDS when the pair changes.  Occupancy as k_adjoint.
 template <int MM, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(MM >= 10 ? 3 : 4))) void k_adjoint_batch(
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<MM>()))) void k_adjoint_batch(
     const PairDesc* __restrict__ pairs, const BatchTile* __restrict__ tiles,
     const unsigned* __restrict__ queueStart, unsigned* __restrict__ counters) {
     extern __shared__ att_pair_params s_dyn[];
