@@ -504,8 +504,10 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
 // (168 VGPRs).  At 4 their per-lane mode amplitudes and gradients spill twice
 // the scratch (252-332 against 84-168 B per lane); at 3 a 4000/1000-bead
 // pair with 10 + 10 modes scores 10 % faster and the 64-pair batched replay
-// takes 2.3 % less time.  The 0- and 5-mode kernels stay at 4 (3 measured
-// slower).  Occupancy changes no arithmetic: results are the same bits.
+// takes 2.3 % less time.  The 0- and 5-mode kernels stay at 4: at 3 (and, for
+// rigid bodies, at 5) they measured slower, e.g. a 1000/300-bead rigid pair
+// 3.58 and 4.44 ms against 3.05 ms for 100k poses.  Occupancy changes no
+// arithmetic: results are the same bits.
 template <int MM>
 constexpr int adjoint_waves_per_simd() { return MM >= 10 ? 3 : 4; }
 
