@@ -89,6 +89,10 @@ def build_service():
         _run([HIPCC] + HIP_FLAGS + ["-shared", "-I", os.path.join(CSRC, "hip"),
                                     os.path.join(tdir, "va13_host.cpp"), "-o",
                                     os.path.join(bin_dir, "libva13_host.so")])
+        # the device sine/cosine, compiled for the host beside the C library
+        _run([HIPCC] + HIP_FLAGS + ["-shared", "-I", os.path.join(CSRC, "hip"),
+                                    os.path.join(tdir, "trig_host.cpp"), "-o",
+                                    os.path.join(bin_dir, "libtrig_host.so"), "-pthread"])
     return out
 
 

@@ -18,11 +18,11 @@
 // (the grid energy is sum_a ipon (rc S_a - ac S6_a) when the plateau exceeds
 // every rmin).
 //
-// k_adjoint: persistent, one 256-thread workgroup per pose at a time; per-pose
-// values go through wave butterflies, parameter derivatives accumulate in LDS
-// across the workgroup's poses and are written once per workgroup, then
-// k_pgrad_groups and k_pgrad_reduce add the per-workgroup rows in a fixed
-// order.
+// k_adjoint: persistent, one workgroup of 1, 2 or 4 waves per pose at a time;
+// per-pose values go through wave butterflies, parameter derivatives
+// accumulate in LDS (one slice per wave) across the workgroup's poses and are
+// written once per workgroup, then k_pgrad_groups and k_pgrad_reduce add the
+// per-workgroup rows in a fixed order.
 #include <algorithm>
 
 #include "device.h"
@@ -55,7 +55,7 @@ __device__ __forceinline__ void trilin_grad(const DGrid& g, float px, float py, 
 }
 
 struct PGrad {
-    float* s;                     // LDS accumulator, T*T*3, or nullptr when not requested
+    float* s;                     // this wave's LDS slice, T*T*3, or nullptr when not requested
     const att_pair_params* tab;   // the LDS copy of the pair table
     unsigned T;
     float w;                      // upstream weight of the current pose
@@ -70,8 +70,8 @@ constexpr unsigned kBasisStep = 2;
 // Grid term of one ligand bead on one grid: energy (the forward's per-bead
 // value), gradient, and the basis-grid parameter derivatives.
 __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, float px, float py,
-                                              float pz, unsigned mtype, float charge, float3& G,
-                                              const PGrad& pg) {
+                                              float pz, unsigned mtype, unsigned type, float charge,
+                                              float3& G, const PGrad& pg) {
     const unsigned ix = grid_index(px, g.lox, g.dVox_inv, g.nx);
     const unsigned iy = grid_index(py, g.loy, g.dVox_inv, g.ny);
     const unsigned iz = grid_index(pz, g.loz, g.dVox_inv, g.nz);
@@ -87,7 +87,10 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
         pot.w = pot.w + el.w;
         G.x += charge * dEl.x; G.y += charge * dEl.y; G.z += charge * dEl.z;
     }
-    if (pg.s && b.oct && mtype <= pg.T) {
+    // The grid of mapped type mtype holds the bead type's column of the pair
+    // table (the grid alphabet maps type -> mtype), so the derivatives go to
+    // the entries (a, type).
+    if (pg.s && b.oct && type - 1u < pg.T) {
         const float mnx = (float)(int)ix * g.dVox + g.lox;
         const float mny = (float)(int)iy * g.dVox + g.loy;
         const float mnz = (float)(int)iz * g.dVox + g.loz;
@@ -115,7 +118,7 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
                                  s1.y * w5 + s1.z * w6 + s1.w * w7) * g.vol_inv;
                 const float S6 = (t0.x * w0 + t0.y * w1 + t0.z * w2 + t0.w * w3 + t1.x * w4 +
                                   t1.y * w5 + t1.z * w6 + t1.w * w7) * g.vol_inv;
-                const unsigned k = (a0 + j) * pg.T + (mtype - 1);
+                const unsigned k = (a0 + j) * pg.T + (type - 1);
                 const float iv = (float)pg.tab[k].ipon * pg.w;
                 atomicAdd(&pg.s[3 * k], iv * S);
                 atomicAdd(&pg.s[3 * k + 1], -iv * S6);
@@ -132,7 +135,7 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
 __device__ __forceinline__ float grid_adj(const DGrid& inner, const DGrid& outer,
                                           const DBasis& bin, const DBasis& bout, const DPar& par,
                                           float px, float py, float pz, unsigned mtype,
-                                          float charge, float3& G, const PGrad& pg) {
+                                          unsigned type, float charge, float3& G, const PGrad& pg) {
     G = make_float3(0.f, 0.f, 0.f);
     if (mtype == 0) return 0.f;
     const bool in = !grid_out(inner, px, py, pz);
@@ -151,7 +154,7 @@ __device__ __forceinline__ float grid_adj(const DGrid& inner, const DGrid& outer
     DBasis b{};
     b.oct = in ? bin.oct : bout.oct;
     b.nb = in ? bin.nb : bout.nb;
-    return grid_adj_one(g, b, px, py, pz, mtype, charge, G, pg);
+    return grid_adj_one(g, b, px, py, pz, mtype, type, charge, G, pg);
 }
 
 // LJ derivatives with respect to (rc, ac, emin) at squared distance dr2.
@@ -165,338 +168,257 @@ __device__ __forceinline__ void lj_dparam(float dr2, float dr2_inv, const att_pa
     dem = dr2 < p.rmin2 ? (float)p.ipon - 1.f : 0.f;
 }
 
-// Neighbour-list cell of a ligand bead: {count, start} of its list, {0, 0}
-// when the bead has no neighbour-list term (same bounds and index as nl_bead).
-__device__ __forceinline__ uint2 nl_cell(const DNL& nl, float px, float py, float pz,
-                                         unsigned ligType) {
-    if (ligType == 0) return make_uint2(0u, 0u);
-    if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
-          (pz >= nl.loz && pz <= nl.hiz)))
-        return make_uint2(0u, 0u);
-    const int ix = (int)roundf((px - nl.lox) * nl.dVox_inv);
-    const int iy = (int)roundf((py - nl.loy) * nl.dVox_inv);
-    const int iz = (int)roundf((pz - nl.loz) * nl.dVox_inv);
-    return nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
-}
-
-// Workgroup-shared state of one pose (static LDS); WAVES waves per pose,
-// 64 * WAVES queued ligand beads per pair round.
-template <int WAVES>
-struct PairLDS {
-    static constexpr unsigned kSlots = 64u * WAVES;
-    float4 p[kSlots];         // transformed bead position, charge
-    float4 x[kSlots];         // deformed bead coordinates x~, bead index | type << 16
-    unsigned start[kSlots];   // first list entry
-    unsigned end[kSlots];     // inclusive prefix of the pair counts
-    float4 G[kSlots];         // sum of pair forces on the bead
-    float4 head[WAVES];       // a wave's share of the bead it starts inside
-    unsigned scan[WAVES];
-    unsigned qn;
-    float red[WAVES][13 + 2 * ATT_MODES_MAX];
-};
-
-// Adjoint of one pose by one workgroup of WAVES waves.
-//  B1   every ligand bead: transform, grid term with its exact gradient (and
-//       the basis-grid parameter derivatives), chain rule; beads with a
-//       neighbour list are recorded with their {count, start}.
-//  Q    the recorded beads are compacted in bead order (block scan).
-//  B2   pair rounds of up to kSlots beads: the pairs of the round are
-//       flattened (prefix sum of counts) and split into WAVES contiguous
-//       ranges, one per wave, walked in chunks of 64 consecutive pairs: list
-//       reads are coalesced and every lane has work until the range ends.
-//       Receptor beads are deformed on the fly from their padded mode row
-//       (one cache line).  Pair forces are summed per bead by a segmented
-//       wave reduction; a bead split between two waves gets the later wave's
-//       share through `head`, added in wave order, so results are
-//       deterministic.  Then the chain rule for the NL force of each bead.
-template <int MM, int WAVES>
-__device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, PairLDS<WAVES>& S,
-                                             char* dyn, const att_pair_params* tab, float* s_pg) {
-    const DProt& rec = P.rec;
-    const DProt& lig = P.lig;
+// The neighbour-list pairs of one ligand bead at p (charge q, type lt) over
+// its list d = {numEl, idx}, in list order:
+//  eAcc  the pair energies folded as NLPotForce folds them (+LJ(r), -LJ(P),
+//        +C(r), -C(P), model/neighborlist.h:66-158): the reference's bits;
+//  gAcc  the sum of the pair forces (LJ and Coulomb at r; the plateau terms
+//        are constant in position);
+//  gR    += force . (receptor mode vector) per pair (dE/d receptor amplitude
+//        through the receptor's deformation);
+//  the parameter derivatives into this wave's LDS slice.
+// Receptor beads are deformed from their padded mode row (one cache line).
+template <int MM>
+__device__ __forceinline__ void adj_walk(const PairDesc& P, const att_pair_params* tab,
+                                         const PGrad& pg, float px, float py, float pz, float q,
+                                         unsigned lt, uint2 d, const float* ampR, float* gR,
+                                         float& eAcc, float3& gAcc) {
+    constexpr int MA = MM > 0 ? MM : 1;
     const DNL& nl = P.nl;
     const DPar& par = P.par;
-    uint2* s_nl = reinterpret_cast<uint2*>(dyn + adjoint_nl_offset(P.tabSlots));
-    unsigned short* s_queue = reinterpret_cast<unsigned short*>(dyn + adjoint_queue_offset(P.tabSlots, lig.n));
-    constexpr unsigned kSlots = PairLDS<WAVES>::kSlots;
-    const unsigned lane = threadIdx.x & (kWave - 1);
-    const unsigned wave = threadIdx.x / kWave;
+    const float4* __restrict__ rpos = P.rec.pos;
+    const float4* __restrict__ rmodesT = P.rec.modesT;
+    const unsigned Mr = MM > 0 ? P.rec.M : 0, rowT = P.rec.rowT;
+    const unsigned lt1 = lt - 1;
+    const uint32_t* lst = nl.list + d.y;
+    eAcc = 0.f;
+    gAcc = make_float3(0.f, 0.f, 0.f);
+    unsigned nxt = d.x ? lst[0] : 0u;
+    for (unsigned jj = 0; jj < d.x; ++jj) {
+        const unsigned j = nxt;
+        if (jj + 1 < d.x) nxt = lst[jj + 1];  // next entry in flight
+        const float4 rx = rpos[j];
+        const unsigned rt = P.rec.type[j];
+        float mt[MM > 0 ? 4 * ((3 * MA + 3) / 4) : 4];
+        float yx = rx.x, yy = rx.y, yz = rx.z;
+        if (MM > 0) {
+            constexpr unsigned Q = (3 * MA + 3) / 4;
+            const float4* row = rmodesT + (size_t)j * rowT;
+#pragma unroll
+            for (unsigned qq = 0; qq < Q; ++qq) {
+                const float4 v = qq < rowT ? row[qq] : make_float4(0.f, 0.f, 0.f, 0.f);
+                mt[4 * qq] = v.x; mt[4 * qq + 1] = v.y; mt[4 * qq + 2] = v.z; mt[4 * qq + 3] = v.w;
+            }
+            // the deformation of deform(), same order of operations
+#pragma unroll
+            for (unsigned m = 0; m < (unsigned)MA; ++m)
+                if (m < Mr) {
+                    const float a = ampR[m];
+                    yx = yx + a * mt[3 * m];
+                    yy = yy + a * mt[3 * m + 1];
+                    yz = yz + a * mt[3 * m + 2];
+                }
+        }
+        const float ddx = px - yx, ddy = py - yy, ddz = pz - yz;
+        const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (!(dr2 <= nl.P2) || rt - 1u >= par.T) continue;
+        const float dr2_inv = (float)(1.0 / (double)dr2);
+        const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
+        const unsigned kk = par.T * (rt - 1) + lt1;
+        const att_pair_params pp = tab[kk];
+        float Fx, Fy, Fz, ve, vx, vy, vz;
+        lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, Fx, Fy, Fz, ve);
+        eAcc += ve;
+        lj(nl.P2, nl.P2_inv, 0.f, 0.f, 0.f, pp, par.shape, vx, vy, vz, ve);
+        eAcc -= ve;
+        if (q != 0.f) {
+            const float qlr = q * rx.w * par.ffelec;
+            if (fabsf(qlr) > 0.001f) {
+                coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
+                eAcc += ve;
+                Fx += vx; Fy += vy; Fz += vz;
+                coulomb(nl.P2_inv, 0.f, 0.f, 0.f, qlr, par.dielec, vx, vy, vz, ve);
+                eAcc -= ve;
+            }
+        }
+        gAcc.x += Fx; gAcc.y += Fy; gAcc.z += Fz;
+        if (MM > 0) {
+#pragma unroll
+            for (unsigned m = 0; m < (unsigned)MA; ++m)
+                if (m < Mr) gR[m] += Fx * mt[3 * m] + Fy * mt[3 * m + 1] + Fz * mt[3 * m + 2];
+        }
+        if (pg.s) {
+            float drc, dac, dem, drcP, dacP, demP;
+            lj_dparam(dr2, dr2_inv, pp, par.shape, drc, dac, dem);
+            lj_dparam(nl.P2, nl.P2_inv, pp, par.shape, drcP, dacP, demP);
+            atomicAdd(&pg.s[3 * kk], pg.w * (drc - drcP));
+            atomicAdd(&pg.s[3 * kk + 1], pg.w * (dac - dacP));
+            if (dem != demP) atomicAdd(&pg.s[3 * kk + 2], pg.w * (dem - demP));
+        }
+    }
+}
+
+// The workgroup-shared state of one pose (static LDS), WAVES waves per pose:
+// the ligand is taken in super-chunks of 256 beads per wave (26 B of LDS per
+// bead: four workgroups of two waves, or eight of one, per CU).
+template <int WAVES>
+struct PoseLDS {
+    static constexpr unsigned kSuper = 256u * WAVES;
+    float4 g[kSuper];     // per bead: dE/dp (grid, then minus the pair forces); energy
+    uint2 nlc[kSuper];    // neighbour-list {count, start}
+    unsigned short q[kSuper];
+    unsigned cc[queue_counters(kSuper)];
+    float red[WAVES][ATT_MODES_MAX];
+    float e;
+};
+
+// Adjoint of one pose by one workgroup of WAVES waves, the ligand in
+// super-chunks of PoseLDS<WAVES>::kSuper beads:
+//  P1  every bead: transform, grid term with its exact gradient (and the
+//      basis-grid parameter derivatives), neighbour-list cell;
+//  P2  beads with a list, longest lists first (class_queue);
+//  P3  one bead per lane walks its list (adj_walk); the bead's energy becomes
+//      grid + pair fold, the reference's per-bead value;
+//  P4  the energy summed in bead order (reducePotForce's order, so the pose
+//      energy is the reference's bits), and the chain rule through p = R x~ + t
+//      by wave 0, bead k on lane k % 64, lane sums reduced by a butterfly.
+// Every sum has a fixed order: results do not depend on timing.
+template <int MM, int WAVES>
+__device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, PoseLDS<WAVES>& S,
+                                             const att_pair_params* tab, float* s_pg) {
+    const DProt& lig = P.lig;
+    const DPar& par = P.par;
+    const unsigned tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    constexpr unsigned nthr = 64u * WAVES;
+    constexpr unsigned eThread = WAVES > 1 ? 64u : 0u;  // sums the energy
     constexpr int MA = MM > 0 ? MM : 1;
-    const float4* __restrict__ rpos = rec.pos;
-    const float4* __restrict__ rmodesT = rec.modesT;
-    const unsigned Mr = MM > 0 ? rec.M : 0, Ml = MM > 0 ? lig.M : 0, rowT = rec.rowT;
-    {
-        __syncthreads();  // LDS of the previous pose is no longer read
-        const float* d = P.dofs + (size_t)P.dofStride * pose;
-        const float tx = d[0], ty = d[1], tz = d[2];
-        const float wp = P.wts ? P.wts[pose] : 1.f;
-        const PGrad pg{s_pg, tab, par.T, wp};
-        float ampR[MA], ampL[MA];
+    const unsigned Ml = MM > 0 ? lig.M : 0, Mr = MM > 0 ? P.rec.M : 0;
+    __syncthreads();  // LDS of the previous pose is no longer read
+    const float* d = P.dofs + (size_t)P.dofStride * pose;
+    const float tx = d[0], ty = d[1], tz = d[2];
+    const float wp = P.wts ? P.wts[pose] : 1.f;
+    const PGrad pg{s_pg ? s_pg + wave * 3 * par.T * par.T : nullptr, tab, par.T, wp};
+    float ampR[MA], ampL[MA];
 #pragma unroll
-        for (int m = 0; m < MA; ++m) {
-            ampR[m] = MM > 0 ? d[6 + m] : 0.f;
-            ampL[m] = MM > 0 ? d[16 + m] : 0.f;
-        }
-        float t[6];
-        pose_trig(d[3], d[4], d[5], lane, t);
-        float R[9];
-        rotmat_from_trig(t[0], t[1], t[2], t[3], t[4], t[5], R);
-        // accumulators: E, dE/dt (3), sum G (x) x~ (9), receptor and ligand modes
-        float acc[13];
-        float gR[MA], gL[MA];
+    for (int m = 0; m < MA; ++m) {
+        ampR[m] = MM > 0 ? d[6 + m] : 0.f;
+        ampL[m] = MM > 0 ? d[16 + m] : 0.f;
+    }
+    float t[6];
+    pose_trig(d[3], d[4], d[5], lane, t);
+    float R[9];
+    rotmat_from_trig(t[0], t[1], t[2], t[3], t[4], t[5], R);
+    // wave 0: dE/dt (3), sum G (x) x~ (9), ligand modes; all waves: receptor modes
+    float acc[12], gR[MA], gL[MA];
 #pragma unroll
-        for (int k = 0; k < 13; ++k) acc[k] = 0.f;
+    for (int k = 0; k < 12; ++k) acc[k] = 0.f;
 #pragma unroll
-        for (int m = 0; m < MA; ++m) { gR[m] = 0.f; gL[m] = 0.f; }
+    for (int m = 0; m < MA; ++m) { gR[m] = 0.f; gL[m] = 0.f; }
+    float esum = 0.f;
 
-        auto place = [&](unsigned i, float4& x, V3& xt) __attribute__((always_inline)) {
-            x = lig.pos[i];
-            BeadModes<MM> md;
-            load_modes(lig, i, md);
-            xt = deform(md, Ml, x.x, x.y, x.z, ampL);
-            V3 p = matvec(R, xt.x, xt.y, xt.z);
-            p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
-            return p;
-        };
-        // Chain rule through p = R x~ + t for one bead's dE/dp = G.
-        auto add_grad = [&](unsigned i, float xx, float xy, float xz, float Gx, float Gy, float Gz)
-            __attribute__((always_inline)) {
-            acc[1] += Gx; acc[2] += Gy; acc[3] += Gz;
-            acc[4] += Gx * xx; acc[5] += Gx * xy; acc[6] += Gx * xz;
-            acc[7] += Gy * xx; acc[8] += Gy * xy; acc[9] += Gy * xz;
-            acc[10] += Gz * xx; acc[11] += Gz * xy; acc[12] += Gz * xz;
-            if (MM > 0) {
-                // G . R m = (R^T G) . m
-                const float gx = R[0] * Gx + R[3] * Gy + R[6] * Gz;
-                const float gy = R[1] * Gx + R[4] * Gy + R[7] * Gz;
-                const float gz = R[2] * Gx + R[5] * Gy + R[8] * Gz;
-                BeadModes<MM> md;
-                load_modes(lig, i, md);
-                mode_dots(md, Ml, -gx, -gy, -gz, gL);  // mode_dots subtracts
-            }
-        };
+    auto deformed = [&](unsigned i, float4& x) __attribute__((always_inline)) {
+        x = lig.pos[i];
+        BeadModes<MM> md;
+        load_modes(lig, i, md);
+        return deform(md, Ml, x.x, x.y, x.z, ampL);
+    };
+    auto place = [&](unsigned i, float4& x) __attribute__((always_inline)) {
+        const V3 xt = deformed(i, x);
+        V3 p = matvec(R, xt.x, xt.y, xt.z);
+        p.x = p.x + tx; p.y = p.y + ty; p.z = p.z + tz;
+        return p;
+    };
 
-        // B1: grid term of every ligand bead; neighbour-list cells recorded.
-        for (unsigned i = threadIdx.x; i < lig.n; i += blockDim.x) {
+    constexpr unsigned kSuper = PoseLDS<WAVES>::kSuper;
+    for (unsigned sc = 0; sc < lig.n; sc += kSuper) {
+        const unsigned m = min(kSuper, lig.n - sc);
+        // P1
+        for (unsigned k = tid; k < m; k += nthr) {
             float4 x;
-            V3 xt;
-            const V3 p = place(i, x, xt);
+            const V3 p = place(sc + k, x);
             float3 G;
-            acc[0] += grid_adj(P.inner, P.outer, P.bin, P.bout, par, p.x, p.y, p.z, lig.mtype[i], x.w, G, pg);
-            add_grad(i, xt.x, xt.y, xt.z, G.x, G.y, G.z);
-            s_nl[i] = nl_cell(nl, p.x, p.y, p.z, lig.type[i]);
+            const float eg = grid_adj(P.inner, P.outer, P.bin, P.bout, par, p.x, p.y, p.z,
+                                      lig.mtype[sc + k], lig.type[sc + k], x.w, G, pg);
+            S.g[k] = make_float4(G.x, G.y, G.z, eg);
+            S.nlc[k] = nl_cell(P.nl, p.x, p.y, p.z, lig.type[sc + k]);
         }
         __syncthreads();
-
-        // Q: beads with listed neighbours, in bead order.
-        {
-            const unsigned per = (lig.n + blockDim.x - 1) / blockDim.x;
-            const unsigned lo = threadIdx.x * per, hi = min(lig.n, lo + per);
-            unsigned cnt = 0;
-            for (unsigned i = lo; i < hi; ++i) cnt += s_nl[i].x != 0u;
-            unsigned total;
-            unsigned o = block_scan(cnt, S.scan, total);
-            for (unsigned i = lo; i < hi; ++i)
-                if (s_nl[i].x != 0u) s_queue[o++] = (unsigned short)i;
-            if (threadIdx.x == 0) S.qn = total;
-        }
-        __syncthreads();
-        const unsigned qn = S.qn;
-
-        // B2: pair rounds.
-        for (unsigned base = 0; base < qn; base += kSlots) {
-            const unsigned ns = min(kSlots, qn - base);
-            unsigned cnt = 0;
-            if (threadIdx.x < ns) {
-                const unsigned i = s_queue[base + threadIdx.x];
+        // P2
+        const unsigned qn = class_queue([&](unsigned k) { return S.nlc[k].x; }, m, S.q, S.cc, tid,
+                                        nthr, [] { __syncthreads(); });
+        // P3
+        for (unsigned q0 = 0; q0 < qn; q0 += nthr) {
+            if (q0 + tid < qn) {
+                const unsigned k = S.q[q0 + tid];
                 float4 x;
-                V3 xt;
-                const V3 p = place(i, x, xt);
-                const uint2 c = s_nl[i];
-                cnt = c.x;
-                S.p[threadIdx.x] = make_float4(p.x, p.y, p.z, x.w);
-                S.x[threadIdx.x] = make_float4(xt.x, xt.y, xt.z, __uint_as_float(i | lig.type[i] << 16));
-                S.start[threadIdx.x] = c.y;
-                S.G[threadIdx.x] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-            unsigned NP;
-            const unsigned ex = block_scan(cnt, S.scan, NP);
-            if (threadIdx.x < ns) S.end[threadIdx.x] = ex + cnt;
-            if (lane == 0) S.head[wave] = make_float4(0.f, 0.f, 0.f, 0.f);
-            __syncthreads();
-
-            // this wave's range of pairs and the slot it starts in
-            const unsigned per = (NP + WAVES - 1) / WAVES;
-            const unsigned rs = min(NP, wave * per), re = min(NP, rs + per);
-            unsigned b0 = 0;
-            {
-                unsigned lo = 0, hi = ns;  // first slot whose end exceeds rs
-                while (lo < hi) {
-                    const unsigned mid = (lo + hi) >> 1;
-                    if (S.end[mid] > rs) hi = mid; else lo = mid + 1;
-                }
-                b0 = lo;
-            }
-            const unsigned firstSlot = b0;
-            const bool headSplit = rs < re && rs > (b0 ? S.end[b0 - 1] : 0u);
-            for (unsigned B = rs; B < re; B += kWave) {
-                const unsigned p = B + lane;
-                const bool valid = p < re;
-                // Slot of every lane: lane j holds the end of slot b0 + j (ends
-                // increase strictly, the first exceeds B); the lane's slot is b0 +
-                // the number of those ends at or below its pair (ends_up_to).
-                const unsigned e = b0 + lane < ns ? S.end[b0 + lane] : 0xFFFFFFFFu;
-                unsigned long long starts;
-                const unsigned s = min(b0 + ends_up_to(e, B, lane, starts), ns - 1u);
-                b0 += (unsigned)__popcll(__ballot(e <= B + kWave));
-
-                float Fx = 0.f, Fy = 0.f, Fz = 0.f;
-                if (valid) {
-                    const float4 bp = S.p[s];
-                    const unsigned meta = __float_as_uint(S.x[s].w);
-                    const unsigned lt1 = (meta >> 16) - 1u;
-                    const unsigned begin = s ? S.end[s - 1] : 0u;
-                    const unsigned j = nl.list[S.start[s] + (p - begin)];
-                    const float4 rx = rpos[j];
-                    const unsigned rt = rec.type[j];
-                    float mt[MM > 0 ? 4 * ((3 * MA + 3) / 4) : 4];
-                    float yx = rx.x, yy = rx.y, yz = rx.z;
-                    if (MM > 0) {
-                        constexpr unsigned Q = (3 * MA + 3) / 4;
-                        const float4* row = rmodesT + (size_t)j * rowT;
-#pragma unroll
-                        for (unsigned q = 0; q < Q; ++q) {
-                            const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                            mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
-                        }
-                        // the deformation of deform(), same order of operations
-#pragma unroll
-                        for (unsigned m = 0; m < (unsigned)MA; ++m)
-                            if (m < Mr) {
-                                const float a = ampR[m];
-                                yx = yx + a * mt[3 * m];
-                                yy = yy + a * mt[3 * m + 1];
-                                yz = yz + a * mt[3 * m + 2];
-                            }
-                    }
-                    const float ddx = bp.x - yx, ddy = bp.y - yy, ddz = bp.z - yz;
-                    const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    if (dr2 <= nl.P2 && rt - 1u < par.T) {
-                        const float dr2_inv = (float)(1.0 / (double)dr2);
-                        const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
-                        const unsigned kk = par.T * (rt - 1) + lt1;
-                        const att_pair_params pp = tab[kk];
-                        float vx, vy, vz, ve;
-                        lj(dr2, dr2_inv, dx, dy, dz, pp, par.shape, Fx, Fy, Fz, ve);
-                        float e2 = ve;
-                        lj(nl.P2, nl.P2_inv, 0.f, 0.f, 0.f, pp, par.shape, vx, vy, vz, ve);
-                        e2 -= ve;
-                        if (bp.w != 0.f) {
-                            const float qlr = bp.w * rx.w * par.ffelec;
-                            if (fabsf(qlr) > 0.001f) {
-                                coulomb(dr2_inv, dx, dy, dz, qlr, par.dielec, vx, vy, vz, ve);
-                                e2 += ve;
-                                Fx += vx; Fy += vy; Fz += vz;
-                                coulomb(nl.P2_inv, 0.f, 0.f, 0.f, qlr, par.dielec, vx, vy, vz, ve);
-                                e2 -= ve;
-                            }
-                        }
-                        acc[0] += e2;
-                        if (MM > 0) {
-#pragma unroll
-                            for (unsigned m = 0; m < (unsigned)MA; ++m)
-                                if (m < Mr) gR[m] += Fx * mt[3 * m] + Fy * mt[3 * m + 1] + Fz * mt[3 * m + 2];
-                        }
-                        if (pg.s) {
-                            float drc, dac, dem, drcP, dacP, demP;
-                            lj_dparam(dr2, dr2_inv, pp, par.shape, drc, dac, dem);
-                            lj_dparam(nl.P2, nl.P2_inv, pp, par.shape, drcP, dacP, demP);
-                            atomicAdd(&pg.s[3 * kk], pg.w * (drc - drcP));
-                            atomicAdd(&pg.s[3 * kk + 1], pg.w * (dac - dacP));
-                            if (dem != demP) atomicAdd(&pg.s[3 * kk + 2], pg.w * (dem - demP));
-                        }
-                    }
-                }
-                // sum of the forces of each run of lanes sharing a slot, in its first lane
-                const unsigned seg = valid ? s : 0xFFFFFFFFu;
-                seg_run_sums(Fx, Fy, Fz, seg, lane);
-                // the first lane of a run: lane 0, or a slot starts at the lane
-                if (valid && (lane == 0 || ((starts >> lane) & 1ull))) {
-                    if (headSplit && seg == firstSlot) {
-                        float4 h = S.head[wave];
-                        h.x += Fx; h.y += Fy; h.z += Fz;
-                        S.head[wave] = h;
-                    } else {
-                        float4 g = S.G[seg];
-                        g.x += Fx; g.y += Fy; g.z += Fz;
-                        S.G[seg] = g;
-                    }
-                }
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                // shares of beads split between waves, in wave order
-                for (unsigned w = 1; w < (unsigned)WAVES; ++w) {
-                    const unsigned ws = min(NP, w * per);
-                    if (ws >= NP) break;
-                    unsigned lo = 0, hi = ns;
-                    while (lo < hi) {
-                        const unsigned mid = (lo + hi) >> 1;
-                        if (S.end[mid] > ws) hi = mid; else lo = mid + 1;
-                    }
-                    if (ws > (lo ? S.end[lo - 1] : 0u)) {
-                        float4 g = S.G[lo];
-                        const float4 h = S.head[w];
-                        g.x += h.x; g.y += h.y; g.z += h.z;
-                        S.G[lo] = g;
-                    }
-                }
-            }
-            __syncthreads();
-            // chain rule for the neighbour-list force of each bead of the round
-            if (threadIdx.x < ns) {
-                const float4 g = S.G[threadIdx.x], xs = S.x[threadIdx.x];
-                add_grad(__float_as_uint(xs.w) & 0xFFFFu, xs.x, xs.y, xs.z, -g.x, -g.y, -g.z);
-            }
-            __syncthreads();  // slot arrays are reused by the next round
-        }
-
-        // Reduction and epilogue.
-#pragma unroll
-        for (int k = 0; k < 13; ++k) acc[k] = wave_sum(acc[k]);
-#pragma unroll
-        for (int m = 0; m < MA; ++m) { gR[m] = wave_sum(gR[m]); gL[m] = wave_sum(gL[m]); }
-        if (lane == 0) {
-#pragma unroll
-            for (int k = 0; k < 13; ++k) S.red[wave][k] = acc[k];
-#pragma unroll
-            for (int m = 0; m < ATT_MODES_MAX; ++m) {
-                S.red[wave][13 + m] = m < MA ? gR[m < MA ? m : 0] : 0.f;
-                S.red[wave][13 + ATT_MODES_MAX + m] = m < MA ? gL[m < MA ? m : 0] : 0.f;
+                const V3 p = place(sc + k, x);
+                float eAcc;
+                float3 gAcc;
+                adj_walk<MM>(P, tab, pg, p.x, p.y, p.z, x.w, lig.type[sc + k], S.nlc[k], ampR, gR,
+                             eAcc, gAcc);
+                float4 g = S.g[k];
+                g.x = g.x - gAcc.x; g.y = g.y - gAcc.y; g.z = g.z - gAcc.z;
+                g.w = g.w + eAcc;  // outLig_E[i] += eAcc
+                S.g[k] = g;
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            float v[13 + 2 * ATT_MODES_MAX];
-#pragma unroll
-            for (int k = 0; k < 13 + 2 * ATT_MODES_MAX; ++k) {
-                float s = S.red[0][k];
-                for (int ww = 1; ww < WAVES; ++ww) s += S.red[ww][k];
-                v[k] = s;
-            }
-            float da[3];
-            torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], v + 4, da);
-            float* o = P.out + (size_t)P.outStride * pose;
-            o[0] = v[0];
-            o[1] = wp * v[1]; o[2] = wp * v[2]; o[3] = wp * v[3];
-            o[4] = wp * da[0]; o[5] = wp * da[1]; o[6] = wp * da[2];
-            if (P.outStride >= 27)
-#pragma unroll
-                for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
-                    o[7 + m] = m < Mr ? wp * v[13 + m] : 0.f;
-                    o[17 + m] = m < Ml ? wp * v[13 + ATT_MODES_MAX + m] : 0.f;
+        // P4
+        if (tid == eThread)
+            for (unsigned k = 0; k < m; ++k) esum += S.g[k].w;
+        if (wave == 0) {
+            for (unsigned k = lane; k < m; k += kWave) {
+                float4 x;
+                const V3 xt = deformed(sc + k, x);
+                const float4 G = S.g[k];
+                acc[0] += G.x; acc[1] += G.y; acc[2] += G.z;
+                acc[3] += G.x * xt.x; acc[4] += G.x * xt.y; acc[5] += G.x * xt.z;
+                acc[6] += G.y * xt.x; acc[7] += G.y * xt.y; acc[8] += G.y * xt.z;
+                acc[9] += G.z * xt.x; acc[10] += G.z * xt.y; acc[11] += G.z * xt.z;
+                if (MM > 0) {
+                    // dE/da_m = G . R m = (R^T G) . m
+                    const float gx = R[0] * G.x + R[3] * G.y + R[6] * G.z;
+                    const float gy = R[1] * G.x + R[4] * G.y + R[7] * G.z;
+                    const float gz = R[2] * G.x + R[5] * G.y + R[8] * G.z;
+                    BeadModes<MM> md;
+                    load_modes(lig, sc + k, md);
+                    mode_dots(md, Ml, -gx, -gy, -gz, gL);  // mode_dots subtracts
                 }
+            }
         }
+        __syncthreads();  // the super-chunk's arrays are reused
+    }
+
+    // Reductions and epilogue.
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[k] = wave_sum(acc[k]);
+#pragma unroll
+        for (int m = 0; m < MA; ++m) gL[m] = wave_sum(gL[m]);
+    }
+#pragma unroll
+    for (int m = 0; m < MA; ++m) gR[m] = wave_sum(gR[m]);
+    if (lane == 0) {
+#pragma unroll
+        for (int m = 0; m < ATT_MODES_MAX; ++m) S.red[wave][m] = m < MA ? gR[m < MA ? m : 0] : 0.f;
+    }
+    if (tid == eThread) S.e = esum;
+    __syncthreads();
+    if (tid == 0) {
+        float da[3];
+        torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], acc + 3, da);
+        float* o = P.out + (size_t)P.outStride * pose;
+        o[0] = S.e;
+        o[1] = wp * acc[0]; o[2] = wp * acc[1]; o[3] = wp * acc[2];
+        o[4] = wp * da[0]; o[5] = wp * da[1]; o[6] = wp * da[2];
+        if (P.outStride >= 27)
+#pragma unroll
+            for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
+                float s = S.red[0][m];
+                for (unsigned w = 1; w < (unsigned)WAVES; ++w) s += S.red[w][m];
+                o[7 + m] = m < Mr ? wp * s : 0.f;
+                o[17 + m] = m < Ml ? wp * gL[m < MA ? m : 0] : 0.f;
+            }
     }
 }
 
@@ -515,13 +437,16 @@ template <int MM, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<MM>()))) void k_adjoint(
     PairDesc P, float* __restrict__ pgPartial) {
     extern __shared__ att_pair_params s_dyn[];
-    __shared__ PairLDS<WAVES> S;
+    __shared__ PoseLDS<WAVES> S;
     char* dyn = reinterpret_cast<char*>(s_dyn);
     const att_pair_params* tab = stage_table(P.par, s_dyn);
-    float* s_pg = pgPartial ? reinterpret_cast<float*>(dyn + adjoint_pg_offset(P.tabSlots, P.lig.n)) : nullptr;
+    // Parameter derivatives: one LDS slice per wave, so that every slice is
+    // updated by one wave only, in program order; the slices are added in
+    // wave order at the end.  Runs therefore give the same bits.
+    float* s_pg = pgPartial ? reinterpret_cast<float*>(dyn + adjoint_pg_offset(P.tabSlots)) : nullptr;
     const unsigned npg = 3 * P.par.T * P.par.T;
     if (s_pg)
-        for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) s_pg[k] = 0.f;
+        for (unsigned k = threadIdx.x; k < npg * WAVES; k += blockDim.x) s_pg[k] = 0.f;
     const unsigned n = P.nDev ? min(P.nDev[0], P.n) : P.n;
     // With a spatial order (and at least 8 workgroups), workgroup b (on XCD
     // b % 8) walks the (b % 8)-th eighth of it; otherwise the poses in input
@@ -535,11 +460,14 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
         end = (unsigned)(((unsigned long long)n * (x + 1)) >> 3);
     }
     for (unsigned i = first; i < end; i += step)
-        adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, dyn, tab, s_pg);
+        adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, tab, s_pg);
     if (s_pg) {
         __syncthreads();
-        for (unsigned k = threadIdx.x; k < npg; k += blockDim.x)
-            pgPartial[(size_t)blockIdx.x * npg + k] = s_pg[k];
+        for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) {
+            float v = s_pg[k];
+            for (unsigned w = 1; w < (unsigned)WAVES; ++w) v += s_pg[w * npg + k];
+            pgPartial[(size_t)blockIdx.x * npg + k] = v;
+        }
     }
 }
 
@@ -554,9 +482,8 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
     const PairDesc* __restrict__ pairs, const BatchTile* __restrict__ tiles,
     const unsigned* __restrict__ queueStart, unsigned* __restrict__ counters) {
     extern __shared__ att_pair_params s_dyn[];
-    __shared__ PairLDS<WAVES> S;
+    __shared__ PoseLDS<WAVES> S;
     __shared__ unsigned s_tile;
-    char* dyn = reinterpret_cast<char*>(s_dyn);
     unsigned cur = 0xFFFFFFFFu;
     const att_pair_params* tab = nullptr;
     for (unsigned q = 0; q < kBatchQueues; ++q) {
@@ -575,25 +502,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
                 cur = tile.pair;
             }
             for (unsigned i = tile.first; i < tile.first + tile.count; ++i)
-                adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, dyn, tab, nullptr);
+                adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, tab, nullptr);
         }
     }
-}
-
-// The wave primitives of the pair loop on given inputs (one wave, diagnostics).
-__global__ __launch_bounds__(64) void k_debug_wave_ops(const float* __restrict__ F,
-                                                       const unsigned* __restrict__ seg,
-                                                       const unsigned* __restrict__ ends, unsigned B,
-                                                       float* __restrict__ sums,
-                                                       unsigned* __restrict__ counts,
-                                                       unsigned long long* __restrict__ starts) {
-    const unsigned lane = threadIdx.x;
-    float Fx = F[lane], Fy = F[64 + lane], Fz = F[128 + lane];
-    seg_run_sums(Fx, Fy, Fz, seg[lane], lane);
-    sums[lane] = Fx; sums[64 + lane] = Fy; sums[128 + lane] = Fz;
-    unsigned long long m;
-    counts[lane] = ends_up_to(ends[lane], B, lane, m);
-    if (lane == 0) starts[0] = m;
 }
 
 __global__ void k_reset_counters(unsigned* counters, unsigned n) {
@@ -667,18 +578,20 @@ __global__ __launch_bounds__(256) void k_basis_octets(const float* __restrict__ 
 }
 
 size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool pgrad,
-                         unsigned* tabSlots) {
+                         int waves, unsigned* tabSlots) {
     (void)rec;
+    (void)lig;
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
-    return adjoint_pg_offset(slots, lig.n) + (pgrad ? sizeof(float) * 3 * (size_t)par.T * par.T : 0);
+    return adjoint_pg_offset(slots) +
+           (pgrad ? sizeof(float) * 3 * (size_t)par.T * par.T * (size_t)waves : 0);
 }
 
 size_t adjoint_static_lds(int waves) {
     switch (waves) {
-        case 1: return sizeof(PairLDS<1>) + 16;
-        case 2: return sizeof(PairLDS<2>) + 16;
-        default: return sizeof(PairLDS<4>) + 16;
+        case 1: return sizeof(PoseLDS<1>) + 16;
+        case 2: return sizeof(PoseLDS<2>) + 16;
+        default: return sizeof(PoseLDS<4>) + 16;
     }
 }
 
@@ -712,14 +625,6 @@ hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsig
     void* args[] = {const_cast<PairDesc*>(&P), &pgPartial};
     return hipLaunchKernel(adjoint_kernel_for(mm, waves, false), dim3(blocks), dim3(64 * waves),
                            args, lds, s);
-}
-
-hipError_t launch_debug_wave_ops(const float* F, const unsigned* seg, const unsigned* ends,
-                                 unsigned B, float* sums, unsigned* counts,
-                                 unsigned long long* starts, hipStream_t s) {
-    hipLaunchKernelGGL(k_debug_wave_ops, dim3(1), dim3(64), 0, s, F, seg, ends, B, sums, counts,
-                       starts);
-    return hipGetLastError();
 }
 
 hipError_t launch_adjoint_batch(unsigned mm, int waves, const PairDesc* pairs, const BatchTile* tiles,

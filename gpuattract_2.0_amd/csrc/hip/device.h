@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "attract_hip.h"
+#include "libm_sincosf.h"
 
 namespace att {
 
@@ -138,17 +139,12 @@ __device__ __forceinline__ void torque_reduce(float cSSI, float cPHI, float sSSI
 }
 
 // The six trig values of a pose, evaluated once per wave: lanes 0..2 take
-// sincos of (ssi, phi, rot) in double and round to float (the correctly
-// rounded single-precision sin/cos), then every lane reads them back.
+// sin and cos of (ssi, phi, rot) with the C library's single-precision values
+// (sincosf_libm, as the reference's std::sin/std::cos(float)), then every lane
+// reads them back.
 __device__ __forceinline__ void pose_trig(float phi, float ssi, float rot, int lane, float* t) {
     float c = 0.f, s = 0.f;
-    if (lane < 3) {
-        const double a = lane == 0 ? (double)ssi : lane == 1 ? (double)phi : (double)rot;
-        double sd, cd;
-        sincos(a, &sd, &cd);
-        c = (float)cd;
-        s = (float)sd;
-    }
+    if (lane < 3) sincosf_libm(lane == 0 ? ssi : lane == 1 ? phi : rot, &s, &c);
     t[0] = __shfl(c, 0);  // cSSI
     t[1] = __shfl(c, 1);  // cPHI
     t[2] = __shfl(s, 0);  // sSSI
@@ -278,33 +274,41 @@ __device__ __forceinline__ void coulomb(float dr2_inv, float dx, float dy, float
     }
 }
 
-// Neighbour-list term of one ligand bead (model/neighborlist.h:47-158 and the
-// receptor reaction of model/neighborlist_modes.h:68-194).  Accessors:
-//   recPos(j)    -> float3 receptor position used for the distance
-//   recType(j)   -> receptor type
-//   recCharge(j) -> receptor charge (only read when the ligand bead is charged)
-//   reaction(j, fx, fy, fz) receives the pair's reaction force (no-op for rigid)
-// List entries are fetched four at a time ahead of use; the pairs are still
-// accumulated one after another in list order, as the reference does.
-// Returns false when the bead is outside the neighbour-list grid.
-template <class RecPos, class RecType, class RecCharge, class Reaction>
-__device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
-                                        const att_pair_params* tab, RecPos recPos,
-                                        RecType recType, RecCharge recCharge, Reaction reaction,
-                                        float px, float py, float pz, unsigned ligType,
-                                        float ligCharge, float4& acc, int* cell,
-                                        unsigned* npairs) {
-    acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (ligType == 0) return false;
+// Neighbour-list cell of a ligand bead (NLGrid bounds and getIndex,
+// model/NLGrid.h:100-131, as NLPotForce tests them, model/neighborlist.h:
+// 47-63): {numEl, idx} of the cell's list, {0, 0} when the bead has no
+// neighbour-list term (type 0 or outside the grid).  cell receives the index.
+__device__ __forceinline__ uint2 nl_cell(const DNL& nl, float px, float py, float pz,
+                                         unsigned ligType, int* cell = nullptr) {
+    if (ligType == 0) return make_uint2(0u, 0u);
     if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
           (pz >= nl.loz && pz <= nl.hiz)))
-        return false;
+        return make_uint2(0u, 0u);
     // NLGrid::getIndex (model/NLGrid.h:100-108): round half away from zero.
     const int ix = (int)roundf((px - nl.lox) * nl.dVox_inv);
     const int iy = (int)roundf((py - nl.loy) * nl.dVox_inv);
     const int iz = (int)roundf((pz - nl.loz) * nl.dVox_inv);
     if (cell) { cell[0] = ix; cell[1] = iy; cell[2] = iz; }
-    const uint2 d = nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
+    return nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
+}
+
+// The pair loop of NLPotForce for one ligand bead (model/neighborlist.h:66-158
+// and the receptor reaction of model/neighborlist_modes.h:68-194) over the
+// list d = {numEl, idx}: every pair in list order, each term added to or
+// subtracted from the bead's accumulators in the reference's order, so the
+// bead's sums are the reference's bits.  Accessors:
+//   recPos(j)    -> float3 receptor position used for the distance
+//   recType(j)   -> receptor type
+//   recCharge(j) -> receptor charge (only read when the ligand bead is charged)
+//   reaction(j, fx, fy, fz) receives the pair's reaction force (no-op for rigid)
+// List entries are fetched four at a time ahead of use.
+template <class RecPos, class RecType, class RecCharge, class Reaction>
+__device__ __forceinline__ float4 nl_walk(const DNL& nl, const DPar& par,
+                                          const att_pair_params* tab, RecPos recPos,
+                                          RecType recType, RecCharge recCharge,
+                                          Reaction reaction, float px, float py, float pz,
+                                          unsigned ligType, float ligCharge, uint2 d,
+                                          unsigned* npairs) {
     float fx = 0.f, fy = 0.f, fz = 0.f, e = 0.f;
     unsigned pairs = 0;
     const unsigned lt1 = ligType - 1;
@@ -379,46 +383,90 @@ __device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
             reaction(nIdx, rfx, rfy, rfz);
         }
     }
-    acc = make_float4(fx, fy, fz, e);
     if (npairs) *npairs = pairs;
+    return make_float4(fx, fy, fz, e);
+}
+
+// Neighbour-list term of one ligand bead: cell, then the pair loop.  Returns
+// false when the bead has no neighbour-list term.
+template <class RecPos, class RecType, class RecCharge, class Reaction>
+__device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
+                                        const att_pair_params* tab, RecPos recPos,
+                                        RecType recType, RecCharge recCharge, Reaction reaction,
+                                        float px, float py, float pz, unsigned ligType,
+                                        float ligCharge, float4& acc, int* cell,
+                                        unsigned* npairs) {
+    acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (npairs) *npairs = 0;
+    if (ligType == 0) return false;
+    if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
+          (pz >= nl.loz && pz <= nl.hiz)))
+        return false;
+    const uint2 d = nl_cell(nl, px, py, pz, ligType, cell);
+    acc = nl_walk(nl, par, tab, recPos, recType, recCharge, reaction, px, py, pz, ligType,
+                  ligCharge, d, npairs);
     return true;
 }
 
-// Number of listed neighbours nl_bead will traverse for a bead (0 when the
-// bead has no neighbour-list term); same bounds and index as nl_bead.
-__device__ __forceinline__ unsigned nl_count(const DNL& nl, float px, float py, float pz,
-                                             unsigned ligType) {
-    if (ligType == 0) return 0;
-    if (!((px >= nl.lox && px <= nl.hix) && (py >= nl.loy && py <= nl.hiy) &&
-          (pz >= nl.loz && pz <= nl.hiz)))
-        return 0;
-    const int ix = (int)roundf((px - nl.lox) * nl.dVox_inv);
-    const int iy = (int)roundf((py - nl.loy) * nl.dVox_inv);
-    const int iz = (int)roundf((pz - nl.loz) * nl.dVox_inv);
-    return nl.desc[nl.nx * (iz * nl.ny + iy) + ix].x;
+// Beads whose neighbour lists are walked one per lane are ordered by list
+// length: nine classes, longest first (>= 64, 48, 36, 28, 22, 16, 11, 6, 1
+// pairs), bead order inside a class, so that the lanes of a wave walk lists of
+// similar length.  The class of a bead with cnt > 0 listed neighbours:
+constexpr unsigned kQueueClasses = 9;
+__device__ __forceinline__ unsigned queue_class(unsigned cnt) {
+    return (unsigned)(cnt < 64u) + (cnt < 48u) + (cnt < 36u) + (cnt < 28u) + (cnt < 22u) +
+           (cnt < 16u) + (cnt < 11u) + (cnt < 6u);
+}
+__host__ __device__ constexpr unsigned queue_counters(unsigned beads) {
+    return kQueueClasses * ((beads + 63) / 64) + 1;
 }
 
-// Bitonic sort of n (a power of two) keys in LDS into descending order, by the
-// whole workgroup.
-__device__ __forceinline__ void bitonic_sort_desc(unsigned* s, unsigned n) {
-    for (unsigned k = 2; k <= n; k <<= 1)
-        for (unsigned j = k >> 1; j > 0; j >>= 1) {
-            for (unsigned i = threadIdx.x; i < n; i += blockDim.x) {
-                const unsigned l = i ^ j;
-                if (l > i) {
-                    const unsigned a = s[i], b = s[l];
-                    const bool swap = ((i & k) == 0) ? (a < b) : (a > b);
-                    if (swap) { s[i] = b; s[l] = a; }
-                }
-            }
-            __syncthreads();
+// The listed beads (cnt(k) > 0) among beads 0..m-1 of a super-chunk, in the
+// class order above, into s_q; returns their number.  Run by nthr threads (a
+// multiple of 64, thread tid) with sync() a barrier over them; s_cc holds
+// queue_counters(m) counters: per (class, chunk of 64 beads) the number of
+// listed beads, then their exclusive prefix in (class, chunk) order.
+template <class Cnt, class Sync>
+__device__ __forceinline__ unsigned class_queue(Cnt cnt, unsigned m, unsigned short* s_q,
+                                                unsigned* s_cc, unsigned tid, unsigned nthr,
+                                                Sync sync) {
+    const unsigned lane = tid & (kWave - 1), wv = tid / kWave, nw = nthr / kWave;
+    const unsigned chunks = (m + kWave - 1) / kWave;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (unsigned ch = wv; ch < chunks; ch += nw) {
+        const unsigned k = ch * kWave + lane;
+        const unsigned c = k < m ? cnt(k) : 0u;
+        const unsigned cls = c ? queue_class(c) : kQueueClasses;
+#pragma unroll
+        for (unsigned q = 0; q < kQueueClasses; ++q) {
+            const unsigned long long b = __ballot(cls == q);
+            if (lane == 0) s_cc[q * chunks + ch] = (unsigned)__popcll(b);
         }
-}
-
-__host__ __device__ inline unsigned pow2_ceil(unsigned v) {
-    unsigned p = 1;
-    while (p < v) p <<= 1;
-    return p;
+    }
+    sync();
+    if (tid == 0) {
+        unsigned sum = 0;
+        for (unsigned e = 0; e < kQueueClasses * chunks; ++e) {
+            const unsigned v = s_cc[e];
+            s_cc[e] = sum;
+            sum += v;
+        }
+        s_cc[kQueueClasses * chunks] = sum;
+    }
+    sync();
+    for (unsigned ch = wv; ch < chunks; ch += nw) {
+        const unsigned k = ch * kWave + lane;
+        const unsigned c = k < m ? cnt(k) : 0u;
+        const unsigned cls = c ? queue_class(c) : kQueueClasses;
+#pragma unroll
+        for (unsigned q = 0; q < kQueueClasses; ++q) {
+            const unsigned long long b = __ballot(cls == q);
+            if (cls == q) s_q[s_cc[q * chunks + ch] + (unsigned)__popcll(b & below)] = (unsigned short)k;
+        }
+    }
+    const unsigned total = s_cc[kQueueClasses * chunks];
+    sync();
+    return total;
 }
 
 // Butterfly sum over the 64 lanes of a wave (same value on every lane).
@@ -458,8 +506,9 @@ __device__ __forceinline__ const att_pair_params* stage_table(const DPar& par,
     return par.tab;
 }
 
-// Byte offset of the per-bead neighbour-list cells behind the pair table.
-__host__ __device__ constexpr size_t adjoint_nl_offset(unsigned tabSlots) {
+// Byte offset of the adjoint's parameter-derivative slices behind the pair
+// table in its dynamic LDS.
+__host__ __device__ constexpr size_t adjoint_pg_offset(unsigned tabSlots) {
     return (sizeof(att_pair_params) * (size_t)tabSlots + 15) & ~(size_t)15;
 }
 
@@ -477,138 +526,6 @@ struct PairDesc {
     const unsigned* nDev;  // when set, the number of poses is read here at launch (<= n)
     const unsigned* order; // when set, a spatial permutation of the poses (XCD ranges)
 };
-
-// Dynamic LDS of a pair: pair table | per-bead {count, start} | queue of
-// bead indices (u16) | parameter-gradient accumulators.
-__host__ __device__ inline size_t adjoint_queue_offset(unsigned tabSlots, unsigned ligN) {
-    return adjoint_nl_offset(tabSlots) + 8 * (size_t)ligN;
-}
-__host__ __device__ inline size_t adjoint_pg_offset(unsigned tabSlots, unsigned ligN) {
-    return adjoint_queue_offset(tabSlots, ligN) + 2 * (((size_t)ligN + 1) & ~(size_t)1);
-}
-
-// Exclusive prefix sum of one value per thread over the workgroup (s_w holds
-// one value per wave); `total` receives the sum.  Contains two barriers.
-__device__ __forceinline__ unsigned block_scan(unsigned v, unsigned* s_w, unsigned& total) {
-    const unsigned lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    unsigned x = v;
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const unsigned y = __shfl_up(x, d);
-        if (lane >= (unsigned)d) x += y;
-    }
-    if (lane == kWave - 1) s_w[wave] = x;
-    __syncthreads();
-    unsigned before = 0, tot = 0;
-    const unsigned nw = blockDim.x / kWave;
-    for (unsigned w = 0; w < nw; ++w) {
-        const unsigned t = s_w[w];
-        before += w < wave ? t : 0u;
-        tot += t;
-    }
-    total = tot;
-    __syncthreads();
-    return before + x - v;
-}
-
-// OR of v over the lanes of each row of 16 (DPP row rotations, which need no
-// shift direction), then over the four rows (row heads by readlane): uniform.
-__device__ __forceinline__ unsigned wave_or(unsigned v) {
-#define ATT_ROR_OR(O) v |= (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x120 + O, 0xF, 0xF, false);
-    ATT_ROR_OR(1)
-    ATT_ROR_OR(2)
-    ATT_ROR_OR(4)
-    ATT_ROR_OR(8)
-#undef ATT_ROR_OR
-    return (unsigned)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
-                      __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
-}
-
-// Lane j holds e_j, the end of the j-th slot of a chunk of 64 pairs starting
-// at B (ends strictly increasing, all above B; past the last slot, ~0u).
-// Returns, for the pair B + lane, the number of ends at or below it: the
-// ends inside (B, B + 64) are marked in one 64-bit mask m (bit e_j - B: a
-// slot starts at that lane), and the lane counts the marks at or below its
-// position.
-__device__ __forceinline__ unsigned ends_up_to(unsigned e, unsigned B, unsigned lane,
-                                               unsigned long long& m) {
-    const unsigned r = e - B;
-    const unsigned mlo = wave_or(r < 32u ? 1u << r : 0u);
-    const unsigned mhi = wave_or(r - 32u < 32u ? 1u << (r - 32u) : 0u);
-    m = ((unsigned long long)mhi << 32) | mlo;
-    const unsigned long long upto = lane == 63u ? ~0ull : (2ull << lane) - 1ull;
-    return (unsigned)__popcll(m & upto);
-}
-
-// Lane i reads lane i + O of its row of 16 (DPP row shift; O = 1, 2, 4, 8):
-// no LDS traffic, unlike a cross-lane shuffle.
-template <int O>
-__device__ __forceinline__ unsigned row_down(unsigned v) {
-    return (unsigned)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x100 + O, 0xF, 0xF, false);
-}
-template <int O>
-__device__ __forceinline__ float row_down(float v) {
-    return __uint_as_float(row_down<O>(__float_as_uint(v)));
-}
-
-// Segmented sums over runs of lanes with equal `seg` (runs are contiguous:
-// seg never decreases across the wave): the first lane of each run receives
-// the sum of the run, in a fixed order.  First the suffix sums inside each
-// row of 16 lanes (DPP), then the partial of every later row whose first lane
-// continues the run, taken from those lanes in row order.
-__device__ __forceinline__ void seg_run_sums(float& Fx, float& Fy, float& Fz, unsigned seg,
-                                             unsigned lane) {
-    const unsigned col = lane & 15u;
-#define ATT_ROW_STEP(O)                                                              \
-    {                                                                                \
-        const float ox = row_down<O>(Fx), oy = row_down<O>(Fy), oz = row_down<O>(Fz); \
-        const unsigned os = row_down<O>(seg);                                        \
-        if (col + O < 16u && os == seg) { Fx += ox; Fy += oy; Fz += oz; }            \
-    }
-    ATT_ROW_STEP(1)
-    ATT_ROW_STEP(2)
-    ATT_ROW_STEP(4)
-    ATT_ROW_STEP(8)
-#undef ATT_ROW_STEP
-    const unsigned row = lane >> 4;
-#pragma unroll
-    for (unsigned r = 1; r < 4; ++r) {
-        const unsigned hs = (unsigned)__builtin_amdgcn_readlane((int)seg, 16 * r);
-        const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fx), 16 * r));
-        const float hy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fy), 16 * r));
-        const float hz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fz), 16 * r));
-        if (row < r && hs == seg) { Fx += hx; Fy += hy; Fz += hz; }
-    }
-}
-
-
-// The same with a fourth summed value (the energy of a pair).
-__device__ __forceinline__ void seg_run_sums(float& Fx, float& Fy, float& Fz, float& Fe,
-                                             unsigned seg, unsigned lane) {
-    const unsigned col = lane & 15u;
-#define ATT_ROW_STEP4(O)                                                              \
-    {                                                                                 \
-        const float ox = row_down<O>(Fx), oy = row_down<O>(Fy), oz = row_down<O>(Fz);  \
-        const float oe = row_down<O>(Fe);                                             \
-        const unsigned os = row_down<O>(seg);                                         \
-        if (col + O < 16u && os == seg) { Fx += ox; Fy += oy; Fz += oz; Fe += oe; }   \
-    }
-    ATT_ROW_STEP4(1)
-    ATT_ROW_STEP4(2)
-    ATT_ROW_STEP4(4)
-    ATT_ROW_STEP4(8)
-#undef ATT_ROW_STEP4
-    const unsigned row = lane >> 4;
-#pragma unroll
-    for (unsigned r = 1; r < 4; ++r) {
-        const unsigned hs = (unsigned)__builtin_amdgcn_readlane((int)seg, 16 * r);
-        const float hx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fx), 16 * r));
-        const float hy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fy), 16 * r));
-        const float hz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fz), 16 * r));
-        const float he = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(Fe), 16 * r));
-        if (row < r && hs == seg) { Fx += hx; Fy += hy; Fz += hz; Fe += he; }
-    }
-}
 
 // Slot of workgroup b among nb in a non-persistent launch such that the
 // workgroups of one XCD (dispatched round-robin: b % 8) take one contiguous
@@ -692,6 +609,59 @@ __device__ __forceinline__ void mode_dots(const BeadModes<MM>& md, unsigned M, f
 #pragma unroll
     for (unsigned m = 0; m < (unsigned)MM; ++m)
         if (m < M) g[m] -= fx * md.x[m] + fy * md.y[m] + fz * md.z[m];
+}
+
+// Barrier over the lanes of one wave (LDS written by the wave is visible to it).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The reference's per-pose reductions of the ligand's bead forces, in bead
+// order, one quantity per lane (lanes 0..12 of a wave): reducePotForce's E
+// and F (model/reduction.h:39-75) and reduceTorque's tensor sum of x_l * f_k
+// over the undeformed coordinates (:80-104) in lane 4 + 3k + l.  Each call
+// adds beads f[0..m) (float4 {fx, fy, fz, E}, LDS), whose undeformed
+// coordinates are pos[first..first+m), to the lane's running sum.
+__device__ __forceinline__ float reduce_beads_ref(float red, unsigned lane, const float4* f,
+                                                  const float4* pos, unsigned first, unsigned m) {
+    if (lane >= 13u) return red;
+    const unsigned fc = lane == 0 ? 3u : lane < 4u ? lane - 1u : (lane - 4u) / 3u;
+    const unsigned xc = lane < 4u ? 0u : (lane - 4u) % 3u;
+    const bool prod = lane >= 4u;
+    const float* ff = reinterpret_cast<const float*>(f);
+    const float* pf = reinterpret_cast<const float*>(pos + first);
+    unsigned k = 0;
+    // the loads of four beads are issued together; the sum stays sequential
+    for (; k + 4 <= m; k += 4) {
+        float fv[4], xv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            fv[u] = ff[4 * (k + u) + fc];
+            xv[u] = prod ? pf[4 * (k + u) + xc] : 1.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) red += xv[u] * fv[u];  // 1 * f = f exactly
+    }
+    for (; k < m; ++k) red += (prod ? pf[4 * k + xc] : 1.f) * ff[4 * k + fc];
+    return red;
+}
+
+// The epilogue of those reductions, by every lane of the wave that ran them:
+// the 13 sums gathered, the force clamped (reducePotForce) and the torque
+// tensor contracted with dR/d(angle) (reduceTorque, commons/TorqueMat.h:
+// 27-37); out = {E, F[3], dE/dang[3]}.
+__device__ __forceinline__ void finish_ref(float red, const float* t, float* out) {
+    float v[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) v[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red), q));
+    float Fx = v[1], Fy = v[2], Fz = v[3];
+    force_clamp(Fx, Fy, Fz);
+    float a3[3];
+    torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], v + 4, a3);
+    out[0] = v[0]; out[1] = Fx; out[2] = Fy; out[3] = Fz;
+    out[4] = a3[0]; out[5] = a3[1]; out[6] = a3[2];
 }
 
 }  // namespace att

@@ -14,12 +14,19 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
                         const unsigned* order, hipStream_t s);
 
-size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots);
+// rlds: the deformed receptor is staged in LDS (else deformed again per pair
+// from its mode rows, for receptors of any size).
+size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool rlds,
+                       unsigned* tabSlots);
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
-                        unsigned n, const unsigned* order, hipStream_t s);
+                        unsigned n, const unsigned* order, bool rlds, hipStream_t s);
+
+// The receptor is staged in LDS when k_modes then needs at most this much
+// (two workgroups per CU or more).
+constexpr size_t kModesStageLds = 64 * 1024;
 
 hipError_t launch_debug_rigid(const DProt& rec, const DProt& lig, const DGrid& inner,
                               const DGrid& outer, const DNL& nl, const DPar& par,
@@ -33,6 +40,7 @@ hipError_t launch_debug_modes(const DProt& rec, const DProt& lig, const DGrid& r
                               float* rdefo, hipStream_t s);
 
 hipError_t launch_debug_rotmats(const float* ang, unsigned n, float* R, hipStream_t s);
+hipError_t launch_debug_sincosf(const float* x, unsigned n, float* sn, float* cs, hipStream_t s);
 
 hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s);
 
@@ -40,7 +48,7 @@ hipError_t launch_octets(const float4* grid, float4* oct, unsigned nx, unsigned 
                          unsigned ngrids, hipStream_t s);
 
 size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool pgrad,
-                         unsigned* tabSlots);
+                         int waves, unsigned* tabSlots);
 // Static LDS of the adjoint kernels with `waves` waves per pose, and how many
 // of their workgroups one CU holds with `lds` bytes of dynamic LDS.
 size_t adjoint_static_lds(int waves);
@@ -76,11 +84,6 @@ struct MinRun {
 hipError_t launch_min_start(const MinRun& R, const float* dofs, hipStream_t s);
 hipError_t launch_min_step(const MinRun& R, const unsigned* listIn, const unsigned* countIn,
                            const float* reqIn, const float* res, hipStream_t s);
-
-// One wave: the adjoint's segmented run sums and slot counts (diagnostics).
-hipError_t launch_debug_wave_ops(const float* F, const unsigned* seg, const unsigned* ends,
-                                 unsigned B, float* sums, unsigned* counts,
-                                 unsigned long long* starts, hipStream_t s);
 
 // Zero-fills n floats with a kernel (graph-capturable, unlike hipMemsetAsync).
 hipError_t launch_fill_zero(float* p, unsigned n, hipStream_t s);

@@ -560,15 +560,18 @@ att_status att_score_rigid(att_ctx* c, att_handle hr, att_handle hl, att_handle 
 }
 
 static att_status check_modes(att_ctx* c, ProtObj* rec, ProtObj* lig, GridObj* gr, GridObj* gl,
-                              ParObj* p) {
+                              ParObj* p, bool* rlds) {
     att_status st;
     if ((st = check_pair(c, rec, lig, gr, p)) != ATT_OK) return st;
     if (rec->maxMType >= gl->inner.ngrids || rec->maxMType >= gl->outer.ngrids)
         return fail(c, ATT_ERR_INVALID, "receptor mapped type has no ligand grid");
     if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
-    const size_t lds = modes_lds_bytes(rec->d, lig->d, p->d, nullptr) + sizeof(float) * 4 * 33 + 16;
-    if (lds > c->maxLdsPerBlock)
-        return fail(c, ATT_ERR_UNSUPPORTED, "receptor too large for the LDS-staged modes kernel");
+    // static LDS: the receptor mode-gradient rows of the four waves
+    const size_t stat = sizeof(float) * 4 * ATT_MODES_MAX + 16;
+    *rlds = modes_lds_bytes(rec->d, lig->d, p->d, true, nullptr) + stat <=
+            std::min(kModesStageLds, c->maxLdsPerBlock);
+    if (modes_lds_bytes(rec->d, lig->d, p->d, *rlds, nullptr) + stat > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "pair table too large for the modes kernel's LDS");
     return ATT_OK;
 }
 
@@ -587,14 +590,15 @@ att_status att_score_modes_dev(att_ctx* c, att_handle hr, att_handle hl, att_han
     if ((st = resolve(c, hr, hl, hgr, hp, &rec, &lig, &gr, &p)) != ATT_OK) return st;
     GridObj* gl = lookup<GridObj>(c, hgl, Kind::Grid);
     if (!gl) return fail(c, ATT_ERR_HANDLE, "unknown ligand grid handle");
-    if ((st = check_modes(c, rec, lig, gr, gl, p)) != ATT_OK) return st;
+    bool rlds = true;
+    if ((st = check_modes(c, rec, lig, gr, gl, p, &rlds)) != ATT_OK) return st;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const unsigned* order = nullptr;
     if ((st = pose_order(c, reinterpret_cast<const float*>(d_dofs), 26, n, kForwardOrderMin, s, &order)) != ATT_OK)
         return st;
     ATT_HIP(c, launch_modes(rec->d, lig->d, gr->inner, gr->outer, gr->nl, gl->inner, gl->outer,
                             p->d, reinterpret_cast<const float*>(d_dofs),
-                            reinterpret_cast<float*>(d_out), (unsigned)n, order, s));
+                            reinterpret_cast<float*>(d_out), (unsigned)n, order, rlds, s));
     if (order) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
@@ -669,8 +673,8 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     if (pgrad && (!g->bin.oct || g->bin.nb != 2 * p->d.T))
         return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
     unsigned slots = 0;
-    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, &slots);
     const int kWaves = pair_waves(lig->d.n);
+    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, kWaves, &slots);
     if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
@@ -706,6 +710,25 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
                                        reinterpret_cast<double*>(static_cast<char*>(c->d_pg) + rowBytes),
                                        d_pgrad, s));
     if (scratch) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
+    return ATT_OK;
+}
+
+att_status att_param_grad_rmin_emin(const att_pair_params* table, unsigned numTypes, int potShape,
+                                    const float* pgrad, float* out) {
+    if (!table || !pgrad || !out || (potShape != 8 && potShape != 12)) return ATT_ERR_INVALID;
+    // rc = -kr emin rmin^s, ac = -ka emin rmin^6
+    const double q = 1.25992105;  // the float constant of the 12-6 derivation, as a double
+    const double s = potShape == 8 ? 8.0 : 12.0;
+    const double kr = potShape == 8 ? 3.0 : 4.0 / (q * q * q * q * q * q);
+    const double ka = potShape == 8 ? 4.0 : 4.0 / (q * q * q);
+    for (size_t k = 0; k < (size_t)numTypes * numTypes; ++k) {
+        const double r2 = table[k].rmin2 > 0.f ? (double)table[k].rmin2 : 0.0;
+        const double r = std::sqrt(r2), em = (double)table[k].emin;
+        const double gr = pgrad[3 * k], ga = pgrad[3 * k + 1], ge = pgrad[3 * k + 2];
+        const double rs1 = std::pow(r, s - 1.0), r5 = r2 * r2 * r;
+        out[2 * k] = (float)(gr * (-kr * em * s * rs1) + ga * (-ka * em * 6.0 * r5));
+        out[2 * k + 1] = (float)(gr * (-kr * rs1 * r) + ga * (-ka * r5 * r) + ge);
+    }
     return ATT_OK;
 }
 
@@ -830,32 +853,24 @@ att_status att_debug_rotmats(att_ctx* c, const float* angles, size_t n, float* R
     return ATT_OK;
 }
 
-att_status att_debug_wave_ops(att_ctx* c, const float* F, const unsigned* seg, const unsigned* ends,
-                              unsigned B, float* sums, unsigned* counts, uint64_t* starts) {
-    if (!c) return ATT_ERR_INVALID;
-    if (!F || !seg || !ends || !sums || !counts || !starts)
-        return fail(c, ATT_ERR_INVALID, "null argument");
+att_status att_debug_sincosf(att_ctx* c, const float* x, size_t n, float* sn, float* cs) {
+    if (!c || (n && (!x || !sn || !cs))) return fail(c, ATT_ERR_INVALID, "null argument");
+    if (n == 0) return ATT_OK;
+    if (n > (1u << 30)) return fail(c, ATT_ERR_UNSUPPORTED, "too many arguments");
     ATT_HIP(c, hipSetDevice(c->device));
     Tmp t;
-    float *dF, *dS;
-    unsigned *dSeg, *dE, *dC;
-    unsigned long long* dM;
-    ATT_HIP(c, t.alloc(&dF, 192));
-    ATT_HIP(c, t.alloc(&dS, 192));
-    ATT_HIP(c, t.alloc(&dSeg, 64));
-    ATT_HIP(c, t.alloc(&dE, 64));
-    ATT_HIP(c, t.alloc(&dC, 64));
-    ATT_HIP(c, t.alloc(&dM, 1));
-    ATT_HIP(c, hipMemcpy(dF, F, sizeof(float) * 192, hipMemcpyHostToDevice));
-    ATT_HIP(c, hipMemcpy(dSeg, seg, sizeof(unsigned) * 64, hipMemcpyHostToDevice));
-    ATT_HIP(c, hipMemcpy(dE, ends, sizeof(unsigned) * 64, hipMemcpyHostToDevice));
-    ATT_HIP(c, launch_debug_wave_ops(dF, dSeg, dE, B, dS, dC, dM, c->stream));
+    float *dx, *ds, *dc;
+    ATT_HIP(c, t.alloc(&dx, n));
+    ATT_HIP(c, t.alloc(&ds, n));
+    ATT_HIP(c, t.alloc(&dc, n));
+    ATT_HIP(c, hipMemcpy(dx, x, sizeof(float) * n, hipMemcpyHostToDevice));
+    ATT_HIP(c, launch_debug_sincosf(dx, (unsigned)n, ds, dc, c->stream));
     ATT_HIP(c, hipStreamSynchronize(c->stream));
-    ATT_HIP(c, hipMemcpy(sums, dS, sizeof(float) * 192, hipMemcpyDeviceToHost));
-    ATT_HIP(c, hipMemcpy(counts, dC, sizeof(unsigned) * 64, hipMemcpyDeviceToHost));
-    ATT_HIP(c, hipMemcpy(starts, dM, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(sn, ds, sizeof(float) * n, hipMemcpyDeviceToHost));
+    ATT_HIP(c, hipMemcpy(cs, dc, sizeof(float) * n, hipMemcpyDeviceToHost));
     return ATT_OK;
 }
+
 
 att_status att_sync(att_ctx* c, void* stream) {
     if (!c) return ATT_ERR_INVALID;
@@ -1155,7 +1170,7 @@ att_status batch_fill(att_ctx* c, att_batch* b, const att_batch_pair* in, unsign
         if (q.n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one pair");
         if (q.n && (!q.d_dofs || !q.d_out)) return fail(c, ATT_ERR_INVALID, "null buffer");
         unsigned slots = 0;
-        const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
+        const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, 1, &slots);
         if (dyn + adjoint_static_lds(4) > c->maxLdsPerBlock)
             return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
         PairDesc P{};
@@ -1400,7 +1415,7 @@ att_status att_minjob_create(att_ctx* c, att_handle hr, att_handle hl, att_handl
     if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
     std::unique_ptr<att_minjob> j(new att_minjob);
     unsigned slots = 0;
-    j->dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, &slots);
+    j->dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, false, 1, &slots);
     const int kWaves = pair_waves(lig->d.n);  // the single-pair adjoint's shape: same results
     j->waves = kWaves;
     if (j->dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)

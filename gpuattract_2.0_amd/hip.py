@@ -93,6 +93,19 @@ def param_table(p):
     return tab
 
 
+def param_grad_rmin_emin(params, pgrad):
+    """(T*T, 2) {dE/drmin, dE/demin} from the (T*T, 3) {dE/drc, dE/dac,
+    dE/demin} parameter gradient (att_param_grad_rmin_emin: the chain rule
+    through the table's derivation, fileIO/readFile.cpp:479-513)."""
+    tab = param_table(params)
+    pg = np.ascontiguousarray(pgrad, np.float32).reshape(-1, 3)
+    out = np.zeros((pg.shape[0], 2), np.float32)
+    st = lib().att_param_grad_rmin_emin(_p(tab), params.T, params.shape, _p(pg), _p(out))
+    if st != 0:
+        raise AttError(st, "att_param_grad_rmin_emin: invalid arguments")
+    return out
+
+
 class AttError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -125,7 +138,8 @@ def lib():
         L.att_debug_beads_rigid.argtypes = [P, H, H, H, H, P, P, P, P, P]
         L.att_debug_beads_modes.argtypes = [P, H, H, H, H, H, P, P, P, P, P, P, P]
         L.att_debug_rotmats.argtypes = [P, P, C.c_size_t, P]
-        L.att_debug_wave_ops.argtypes = [P, P, P, P, C.c_uint, P, P, P]
+        L.att_debug_sincosf.argtypes = [P, P, C.c_size_t, P, P]
+        L.att_param_grad_rmin_emin.argtypes = [P, C.c_uint, C.c_int, P, P]
         L.att_sync.argtypes = [P, P]
         L.att_build_info.restype = C.c_char_p
         L.att_dev_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
@@ -472,21 +486,16 @@ class Context:
         out["cell"] = out["cell"].reshape(n, 9)
         return out
 
-    def wave_ops(self, F, seg, ends, B):
-        """The adjoint's wave primitives on one wave (att_debug_wave_ops):
-        (run sums (3, 64), counts (64,), starts mask)."""
-        f = np.ascontiguousarray(F, np.float32).reshape(3, 64)
-        sg = np.ascontiguousarray(seg, np.uint32).reshape(64)
-        e = np.ascontiguousarray(ends, np.uint32).reshape(64)
-        sums = np.zeros((3, 64), np.float32)
-        counts = np.zeros(64, np.uint32)
-        starts = np.zeros(1, np.uint64)
-        self._chk(self.L.att_debug_wave_ops(self.h, _p(f), _p(sg), _p(e), int(B), _p(sums),
-                                            _p(counts), _p(starts)))
-        return sums, counts, int(starts[0])
-
     def rotmats(self, angles):
         a = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)
         R = np.zeros((a.shape[0], 9), np.float32)
         self._chk(self.L.att_debug_rotmats(self.h, _p(a), a.shape[0], _p(R)))
         return R
+
+    def sincosf(self, x):
+        """(sin, cos) of float32 values as the kernels evaluate them."""
+        a = np.ascontiguousarray(x, np.float32).ravel()
+        s = np.zeros_like(a)
+        c = np.zeros_like(a)
+        self._chk(self.L.att_debug_sincosf(self.h, _p(a), a.size, _p(s), _p(c)))
+        return s, c

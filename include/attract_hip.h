@@ -201,6 +201,20 @@ att_status att_score_adjoint_dev(att_ctx* ctx, att_handle rec, att_handle lig, a
                                  att_handle params, const float* d_dofs, int modes, size_t n,
                                  const float* d_w, float* d_out, float* d_pgrad, void* stream);
 
+/* The parameter gradient with respect to the pair parameters the table is
+ * derived from (fileIO/readFile.cpp:479-513): per table entry rmin =
+ * sqrt(rmin2) and emin (the table's own field, minus the well depth), of
+ * which rc and ac are functions for the 8-6 shape (rc = -3 emin rmin^8,
+ * ac = -4 emin rmin^6) and the 12-6 shape (rbc^2 = rmin^2 / 1.25992105,
+ * abc = -4 emin, rc = abc rbc^12, ac = abc rbc^6).
+ *   pgrad: numTypes^2 x 3 {dE/drc, dE/dac, dE/demin} (att_score_adjoint);
+ *   out:   numTypes^2 x 2 {dE/drmin, dE/demin}, in double, rounded to float.
+ * The rmin2 branch point of the pair term contributes no derivative (E is
+ * continuous there).  Host arithmetic only; no context, no device.  Returns
+ * ATT_ERR_INVALID for null pointers or a shape other than 8 and 12. */
+att_status att_param_grad_rmin_emin(const att_pair_params* table, unsigned numTypes, int potShape,
+                                    const float* pgrad, float* out);
+
 /* --- diagnostics -------------------------------------------------------------
  * Per-bead intermediates of one pose, computed by the same device functions
  * as the scoring kernels: transformed coordinates (SoA 3n), per-bead
@@ -222,16 +236,11 @@ att_status att_debug_beads_modes(att_ctx* ctx, att_handle rec, att_handle lig, a
  * them (matrixFunctions.h:38-77), for checking them against the host. */
 att_status att_debug_rotmats(att_ctx* ctx, const float* angles /* 3n */, size_t n, float* R /* 9n */);
 
-/* The adjoint's wave primitives on one wave of 64 lanes, for checking them
- * against the host: per lane a force F (SoA: x[64], y[64], z[64]) and a run key
- * seg[64] (non-decreasing); sums (SoA 3x64) receives, in the first lane of
- * every run of equal keys, the sum of the run's forces.  With ends[64] the
- * strictly increasing slot ends of a chunk of 64 pairs starting at B (all
- * above B; ~0u past the last slot), counts[64] receives for each lane the
- * number of ends at or below B + lane and *starts the 64-bit mask of ends
- * inside (B, B + 64).  No reference counterpart. */
-att_status att_debug_wave_ops(att_ctx* ctx, const float* F, const unsigned* seg, const unsigned* ends,
-                              unsigned B, float* sums, unsigned* counts, uint64_t* starts);
+/* Single-precision sine and cosine of n floats as the kernels evaluate them
+ * (the values of the C library's sincosf, which the reference's
+ * std::sin/std::cos(float) calls in matrixFunctions.h:51-60 reach), for
+ * checking them against the host library. */
+att_status att_debug_sincosf(att_ctx* ctx, const float* x, size_t n, float* s, float* c);
 
 att_status att_sync(att_ctx* ctx, void* stream);
 

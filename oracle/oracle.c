@@ -21,19 +21,31 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* Sine and cosine as model/matrixFunctions.h:51-60 evaluates them: the C
+ * library's single-precision functions in the FP32 build (glibc turns each
+ * pair into one sincosf call with the same values), double in the FP64
+ * recompute. */
 #define REAL float
 #define SFX _f32
+#define RCOS(x) cosf(x)
+#define RSIN(x) sinf(x)
 #include "oracle_impl.inc"
 #include "oracle_adj.inc"
 #undef REAL
 #undef SFX
+#undef RCOS
+#undef RSIN
 
 #define REAL double
 #define SFX _f64
+#define RCOS(x) cos(x)
+#define RSIN(x) sin(x)
 #include "oracle_impl.inc"
 #include "oracle_adj.inc"
 #undef REAL
 #undef SFX
+#undef RCOS
+#undef RSIN
 
 
 /* Exact adjoint (oracle_adj.inc), FP32 and FP64 arithmetic. */
@@ -166,6 +178,9 @@ void orc_ratio_lut(float plateau, float* out) {
 }
 
 void orc_euler2rotmat(float phi, float ssi, float rot, float R[9]) { rotmat_f32(phi, ssi, rot, R); }
+void orc_rotmats(const float* ang, size_t n, float* R) {
+    for (size_t i = 0; i < n; ++i) rotmat_f32(ang[3 * i], ang[3 * i + 1], ang[3 * i + 2], R + 9 * i);
+}
 
 void orc_euler2torquemat(float phi, float ssi, float rot, float tm[27]) {
     torquemat_f32(phi, ssi, rot, tm);

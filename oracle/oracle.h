@@ -7,10 +7,8 @@
  * Follows the reference's CPU service (service/CPUEnergyService6D.tpp:62-204,
  * service/CPUEnergyService6DModes.tpp:121-327) and the model headers it calls,
  * in FP32 with the reference's double promotions, plus an FP64 recompute of
- * the same algorithm.  One deliberate deviation: sin/cos are evaluated as the
- * correctly rounded float of the double-precision value, because the
- * reference's std::sin/std::cos(float) results come from the C library
- * (glibc's sinf/cosf are not correctly rounded and differ by CPU variant).
+ * the same algorithm.  Sine and cosine come from the C library's float
+ * functions, as the reference's std::sin/std::cos(float) do.
  */
 #ifndef ATTRACT_ORACLE_H_
 #define ATTRACT_ORACLE_H_
@@ -76,6 +74,7 @@ unsigned orc_ratio_size(float plateau);
 void orc_ratio_lut(float plateau, float* out);
 
 void orc_euler2rotmat(float phi, float ssi, float rot, float R[9]);
+void orc_rotmats(const float* ang /* 3n: phi, ssi, rot */, size_t n, float* R /* 9n */);
 void orc_euler2torquemat(float phi, float ssi, float rot, float tm[27]);
 
 /* Rigid scoring: dof = n*6 {pos, ang}, res = n*7 {E, pos, ang} (Result_6D).

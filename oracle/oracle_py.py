@@ -152,6 +152,9 @@ def ref_lib():
         L.ref_beads_rigid.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(NLGridC),
                                       C.POINTER(RefTableC), P, P, P]
+        L.ref_cells_rigid.restype = C.c_int
+        L.ref_cells_rigid.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                      C.POINTER(IGridC), C.POINTER(NLGridC), P, P, P]
         L.ref_score_modes.restype = C.c_int
         L.ref_score_modes.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
                                       C.POINTER(IGridC), C.POINTER(NLGridC), C.POINTER(IGridC),
@@ -349,6 +352,17 @@ def rotmat(phi, ssi, rot):
     return R
 
 
+def rotmats(ang):
+    """Rotation matrices (n, 9) of n angle triples (phi, ssi, rot)."""
+    L = oracle_lib()
+    a = np.ascontiguousarray(ang, np.float32).reshape(-1, 3)
+    R = np.zeros((a.shape[0], 9), np.float32)
+    L.orc_rotmats.restype = None
+    L.orc_rotmats.argtypes = [P, C.c_size_t, P]
+    L.orc_rotmats(_p(a), a.shape[0], _p(R))
+    return R
+
+
 def torquemat(phi, ssi, rot):
     t = np.zeros(27, np.float32)
     oracle_lib().orc_euler2torquemat(phi, ssi, rot, _p(t))
@@ -386,6 +400,20 @@ def ref_beads_rigid(wl, dof):
                       C.byref(nl_c(wl.rec_nl)), C.byref(table_c(wl.params)),
                       _p(np.ascontiguousarray(dof, np.float32)), _p(xyz), _p(pot))
     return xyz, pot
+
+
+def ref_cells_rigid(wl, dof):
+    """The reference's own grid cells (n, 9) and plateau pair counts (n,) of
+    one rigid pose (layout of beads()["cell"] / ["nl_pairs"])."""
+    L = ref_lib()
+    n = wl.lig.n
+    cell = np.full(9 * n, -1, np.int32)
+    npairs = np.zeros(n, np.uint32)
+    L.ref_cells_rigid(C.byref(body_c(wl.rec)), C.byref(body_c(wl.lig)),
+                      C.byref(igrid_c(wl.rec_inner)), C.byref(igrid_c(wl.rec_outer)),
+                      C.byref(nl_c(wl.rec_nl)), _p(np.ascontiguousarray(dof, np.float32)),
+                      _p(cell), _p(npairs))
+    return cell.reshape(n, 9), npairs
 
 
 def ref_score_modes(wl, dofs=None):

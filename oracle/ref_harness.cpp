@@ -274,6 +274,59 @@ int ref_beads_rigid(const BodyIn* rec, const BodyIn* lig, const GridIn* inner, c
     return 0;
 }
 
+// Grid cells and neighbour-list pair counts of one rigid pose, from the
+// reference's own index functions on its own transformed coordinates:
+// per bead 9 ints {inner x,y,z; outer x,y,z; NL x,y,z} (-1 where the bead has
+// no such lookup: mapped type 0 or out of bounds for the grids as potForce
+// branches (interpolation.h:56-102), type 0 or out of bounds for the list as
+// NLPotForce (neighborlist.h:47-63)), and the number of listed neighbours
+// inside the plateau (the pairs NLPotForce evaluates, neighborlist.h:66-80).
+int ref_cells_rigid(const BodyIn* rec, const BodyIn* lig, const GridIn* inner, const GridIn* outer,
+                    const NLIn* nl, const float* dof, int* cell, unsigned* npairs) {
+    auto pr = make_protein(*rec);
+    auto pl = make_protein(*lig);
+    auto gi = make_igrid(*inner);
+    auto go = make_igrid(*outer);
+    auto gn = make_nlgrid(*nl);
+    const unsigned N = pl->numAtoms();
+    std::vector<float> xyz(3 * N);
+    Vec3<float> pos(dof[0], dof[1], dof[2]), ang(dof[3], dof[4], dof[5]);
+    rotate_translate(pl->xPos(), pl->yPos(), pl->zPos(), pos, ang, N, xyz.data(), xyz.data() + N,
+                     xyz.data() + 2 * N);
+    for (unsigned i = 0; i < N; ++i) {
+        int* c = cell + 9 * i;
+        for (int k = 0; k < 9; ++k) c[k] = -1;
+        npairs[i] = 0;
+        const float x = xyz[i], y = xyz[N + i], z = xyz[2 * N + i];
+        const float3 p = make_float3(x, y, z);
+        if (pl->mappedType()[i] != 0) {
+            if (gi->outOfBounds_byPos(p)) {
+                if (!go->outOfBounds_byPos(p)) {
+                    const int3 ix = go->getIndex(p);
+                    c[3] = ix.x; c[4] = ix.y; c[5] = ix.z;
+                }
+            } else {
+                const int3 ix = gi->getIndex(p);
+                c[0] = ix.x; c[1] = ix.y; c[2] = ix.z;
+            }
+        }
+        if (pl->type()[i] != 0 && !gn->outOfBounds(x, y, z)) {
+            int ix, iy, iz;
+            gn->getIndex(x, y, z, ix, iy, iz);
+            c[6] = ix; c[7] = iy; c[8] = iz;
+            const NeighbourDesc& d = gn->getNeighbourDesc(ix, iy, iz);
+            unsigned np = 0;
+            for (unsigned j = 0; j < d.numEl; ++j) {
+                const unsigned r = gn->getNeighbor(d.idx + j);
+                const float dx = x - pr->xPos()[r], dy = y - pr->yPos()[r], dz = z - pr->zPos()[r];
+                if (!gn->outOfPlateau(dx * dx + dy * dy + dz * dz)) ++np;
+            }
+            npairs[i] = np;
+        }
+    }
+    return 0;
+}
+
 // Modes scoring: model functions in the order of CPUEnergyService6DModes.tpp:121-327,
 // buffers sized by their own protein.  res = 27 floats per pose.
 struct ModesSetup {

@@ -3,9 +3,10 @@
 Runs in the development container only: it needs oracle/_ref/libattract_ref.so,
 compiled by oracle/Makefile from the reference sources.  Each fixture stores
 the generator parameters, a checksum of every input array (so a test can
-prove it regenerated the identical inputs), the DOFs, the reference's outputs
-and, per pose, whether the reference's rotation matrix (C-library sin/cos)
-equals the correctly rounded one the restatement uses.
+prove it regenerated the identical inputs), the DOFs and the reference's
+outputs: per-pose results and, for the first poses, per-bead coordinates,
+forces and energies, and (rigid) the grid cells and neighbour-list pair
+counts from the reference's own index functions.
 
     python tests/golden/make_golden.py
 """
@@ -48,16 +49,7 @@ MODES = [
 ]
 
 
-def r_agree(O, dofs, modes):
-    ok = []
-    for d in dofs:
-        a = d[3:6]
-        same = np.array_equal(O.rotmat(*a), O.ref_rotmat(*a)) and \
-            np.array_equal(O.torquemat(*a), O.ref_torquemat(*a))
-        if modes:
-            same = same and np.array_equal(O.rotmat(*(-a)), O.ref_rotmat(*(-a)))
-        ok.append(same)
-    return np.array(ok)
+BEAD_POSES = 16  # poses with per-bead reference values (rigid)
 
 
 def main():
@@ -69,12 +61,15 @@ def main():
     for spec in RIGID:
         wl = W.make_workload(spec["name"], *spec["args"], **spec["kw"])
         res = O.ref_score_rigid(wl)
-        xyz, pot = O.ref_beads_rigid(wl, wl.dofs[0])
+        k = min(BEAD_POSES, wl.dofs.shape[0])
+        beads = [O.ref_beads_rigid(wl, wl.dofs[i]) for i in range(k)]
+        cells = [O.ref_cells_rigid(wl, wl.dofs[i]) for i in range(k)]
         np.savez_compressed(
             os.path.join(HERE, spec["name"] + ".npz"),
             spec=json.dumps(spec), checksums=json.dumps(input_checksums(W, wl)),
-            dofs=wl.dofs, ref=res, r_agree=r_agree(O, wl.dofs, False), bead_xyz=xyz,
-            bead_pot=pot)
+            dofs=wl.dofs, ref=res, bead_xyz=np.stack([b[0] for b in beads]),
+            bead_pot=np.stack([b[1] for b in beads]), bead_cells=np.stack([c[0] for c in cells]),
+            bead_npairs=np.stack([c[1] for c in cells]))
         print(spec["name"], res.shape)
     for spec in MODES:
         wl = W.make_workload(spec["name"], *spec["args"], **spec["kw"])
@@ -82,8 +77,7 @@ def main():
         np.savez_compressed(
             os.path.join(HERE, spec["name"] + ".npz"),
             spec=json.dumps(spec), checksums=json.dumps(input_checksums(W, wl)),
-            dofs=wl.dofs, ref=res, r_agree=r_agree(O, wl.dofs, True), rec_pot=rec_pot,
-            lig_pot=lig_pot)
+            dofs=wl.dofs, ref=res, rec_pot=rec_pot, lig_pot=lig_pot)
         print(spec["name"], res.shape)
     # Parameter derivation (fileIO/readFile.cpp:439-527) for both shapes.
     out = {}

@@ -41,3 +41,32 @@ def workload(W, rec, lig, params, dofs, inner=None, outer=None, plateau=10.0,
     if lig_inner is not None:
         wl.lig_inner, wl.lig_outer = lig_inner, lig_outer
     return wl
+
+
+def permute_alphabet(W, wl, perm):
+    """The same physical workload with the receptor grids' probe types in the
+    order of a grid alphabet: grid m (1-based) holds probe type perm[m-1]
+    (grid 0, electrostatics, stays first) and the ligand's mapped types follow
+    (mtype = position of its type in the alphabet + 1, as the reference's
+    TypeMap does, model/TypeMap.h:52-60).  Scores, gradients and parameter
+    gradients must not change."""
+    import copy
+    perm = np.asarray(perm, np.int64)
+    T = wl.params.T
+    assert sorted(perm.tolist()) == list(range(1, T + 1))
+    pos = np.zeros(T + 1, np.uint32)
+    pos[perm] = np.arange(1, T + 1, dtype=np.uint32)
+
+    def regrid(g):
+        d = g.data.reshape(g.ngrids, -1)
+        slabs = np.concatenate([d[:1], d[perm]], axis=0)
+        return W.IGrid(g.geom, g.ngrids, np.ascontiguousarray(slabs.ravel()))
+
+    out = copy.copy(wl)
+    lig = copy.copy(wl.lig)
+    t = lig.type
+    lig.mtype = np.where((t >= 1) & (t <= T), pos[np.minimum(t, T)], 0).astype(np.uint32)
+    lig.mtype[wl.lig.mtype == 0] = 0
+    out.lig = lig
+    out.rec_inner, out.rec_outer = regrid(wl.rec_inner), regrid(wl.rec_outer)
+    return out

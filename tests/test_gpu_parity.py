@@ -20,7 +20,57 @@ def ctx():
     c.close()
 
 
+def _host_sincosf(x):
+    """The host C library's sincosf values (the reference's sin/cos)."""
+    import ctypes
+    import os
+    L = ctypes.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "_bin",
+                                 "libtrig_host.so"))
+    L.trig_host_eval.restype = None
+    L.trig_host_eval.argtypes = [ctypes.c_void_p, ctypes.c_size_t] + [ctypes.c_void_p] * 4
+    x = np.ascontiguousarray(x, np.float32)
+    out = [np.zeros_like(x) for _ in range(4)]
+    L.trig_host_eval(x.ctypes.data, x.size, *[o.ctypes.data for o in out])
+    return out[0], out[1]
+
+
+def test_device_sincosf_is_the_c_library(ctx):
+    """The kernels' sin/cos (sincosf_libm on the device) equal the host C
+    library's sincosf bit for bit: a sweep of 4M floats through [-8, 8]
+    (every 256th float pattern below magnitude 8), 1M uniform angles in
+    [-2pi, 2pi], the large-argument reduction, special values and every
+    golden-fixture angle."""
+    import os
+    rng = np.random.default_rng(17)
+    u = np.arange(0, 0x41000000, 256, dtype=np.uint32)
+    sweep = np.concatenate([u, u | np.uint32(0x80000000)]).view(np.float32)
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    fixture = [np.load(os.path.join(gold, "rotmat.npz"))["ang"].ravel()]
+    for f in sorted(os.listdir(gold)):
+        if f.endswith(".npz"):
+            z = np.load(os.path.join(gold, f))
+            if "dofs" in z.files:
+                fixture.append(z["dofs"][:, 3:6].ravel())
+    x = np.concatenate([
+        sweep,
+        rng.uniform(-2 * np.pi, 2 * np.pi, 1000000).astype(np.float32),
+        rng.uniform(-1e6, 1e6, 100000).astype(np.float32),
+        np.array([0.0, -0.0, 1e-45, 2.0 ** -12, 0.75, 119.99, 120.0, 3e38, -3e38], np.float32),
+    ] + fixture)
+    s, c = ctx.sincosf(x)
+    hs, hc = _host_sincosf(x)
+    assert np.array_equal(s.view(np.uint32), hs.view(np.uint32))
+    assert np.array_equal(c.view(np.uint32), hc.view(np.uint32))
+    # infinities and NaN give NaN, as the library does
+    s, c = ctx.sincosf(np.array([np.inf, -np.inf, np.nan], np.float32))
+    assert np.all(np.isnan(s)) and np.all(np.isnan(c))
+
+
 def test_rotmats_bit_exact(ctx, O):
+    """Rotation matrices as the kernels build them equal the oracle's (the
+    reference's formula on the C library's sin/cos) and the reference's own
+    (golden rotmat.npz) bit for bit."""
+    import os
     rng = np.random.default_rng(5)
     ang = np.concatenate([
         rng.uniform(-7, 7, (200000, 3)),
@@ -29,17 +79,10 @@ def test_rotmats_bit_exact(ctx, O):
         np.array([[0, 0, 0], [np.pi, 0, 0], [0, np.pi / 2, 0], [-0.0, 0.0, -0.0]]),
     ]).astype(np.float32)
     R = ctx.rotmats(ang)
-    ref = np.stack([O.rotmat(*a) for a in ang[::97]])
-    assert np.array_equal(R[::97], ref)
-    # all angle triples: against the oracle's formula evaluated in numpy
-    c = lambda v: np.cos(v.astype(np.float64)).astype(np.float32)
-    s = lambda v: np.sin(v.astype(np.float64)).astype(np.float32)
-    phi, ssi, rot = ang[:, 0], ang[:, 1], ang[:, 2]
-    cS, cP, sS, sP, cR, sR = c(ssi), c(phi), s(ssi), s(phi), c(rot), s(rot)
-    assert np.array_equal(R[:, 8], cS)
-    assert np.array_equal(R[:, 2], sS * cP)
-    assert np.array_equal(R[:, 0], cR * (cS * cP) + sR * sP)
-    assert np.array_equal(R[:, 7], -sR * sS)
+    ref = O.rotmats(ang)
+    assert np.array_equal(R.view(np.uint32), ref.view(np.uint32))
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rotmat.npz"))
+    assert np.array_equal(ctx.rotmats(f["ang"]).view(np.uint32), f["R"].view(np.uint32))
 
 
 def _fp64_statement(name, g, o32, o64, sc):
