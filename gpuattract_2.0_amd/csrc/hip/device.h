@@ -301,8 +301,9 @@ __device__ __forceinline__ uint2 nl_cell(const DNL& nl, float px, float py, floa
 //   recType(j)   -> receptor type
 //   recCharge(j) -> receptor charge (only read when the ligand bead is charged)
 //   reaction(j, fx, fy, fz) receives the pair's reaction force (no-op for rigid)
-// List entries are fetched four at a time ahead of use.
-template <class RecPos, class RecType, class RecCharge, class Reaction>
+// List entries are fetched NB at a time ahead of use (NB = 1 where recPos
+// itself loads much, as a receptor deformed from its mode row).
+template <int NB = 4, class RecPos, class RecType, class RecCharge, class Reaction>
 __device__ __forceinline__ float4 nl_walk(const DNL& nl, const DPar& par,
                                           const att_pair_params* tab, RecPos recPos,
                                           RecType recType, RecCharge recCharge,
@@ -315,38 +316,38 @@ __device__ __forceinline__ float4 nl_walk(const DNL& nl, const DPar& par,
     const bool ligCharged = ligCharge != 0.f;  // else q_lig*q_rec*ffelec = 0: no Coulomb term
     const uint32_t* lst = nl.list + d.y;
     // Indices of the next batch are loaded while the current batch computes.
-    unsigned nidx[4];
+    unsigned nidx[NB];
     if (d.x != 0) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) nidx[k] = lst[(unsigned)k < d.x ? k : 0];
+        for (int k = 0; k < NB; ++k) nidx[k] = lst[(unsigned)k < d.x ? k : 0];
     }
-    for (unsigned j0 = 0; j0 < d.x; j0 += 4) {
-        unsigned idx[4];
-        float3 rp[4];
-        unsigned rt[4];
+    for (unsigned j0 = 0; j0 < d.x; j0 += NB) {
+        unsigned idx[NB];
+        float3 rp[NB];
+        unsigned rt[NB];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) idx[k] = nidx[k];
-        if (j0 + 4 < d.x) {
+        for (int k = 0; k < NB; ++k) idx[k] = nidx[k];
+        if (j0 + NB < d.x) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) nidx[k] = lst[j0 + 4 + k < d.x ? j0 + 4 + k : j0 + 4];
+            for (int k = 0; k < NB; ++k) nidx[k] = lst[j0 + NB + k < d.x ? j0 + NB + k : j0 + NB];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { rp[k] = recPos(idx[k]); rt[k] = recType(idx[k]); }
-        // Distances of the four pairs first, then their plateau ratios, so the
-        // four table loads are in flight together.
-        float ddx[4], ddy[4], ddz[4], dr2s[4], ratios[4];
+        for (int k = 0; k < NB; ++k) { rp[k] = recPos(idx[k]); rt[k] = recType(idx[k]); }
+        // Distances of the batch's pairs first, then their plateau ratios, so
+        // the table loads are in flight together.
+        float ddx[NB], ddy[NB], ddz[NB], dr2s[NB], ratios[NB];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NB; ++k) {
             ddx[k] = px - rp[k].x;
             ddy[k] = py - rp[k].y;
             ddz[k] = pz - rp[k].z;
             dr2s[k] = ddx[k] * ddx[k] + ddy[k] * ddy[k] + ddz[k] * ddz[k];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < NB; ++k)
             ratios[k] = (j0 + k < d.x && dr2s[k] <= nl.P2) ? nl.ratio[(int)(dr2s[k] * 10000.f)] : 0.f;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < NB; ++k) {
             if (j0 + k >= d.x) break;
             const unsigned nIdx = idx[k];
             float dx = ddx[k], dy = ddy[k], dz = ddz[k];

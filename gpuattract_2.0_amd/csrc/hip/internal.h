@@ -14,19 +14,13 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
                         const unsigned* order, hipStream_t s);
 
-// rlds: the deformed receptor is staged in LDS (else deformed again per pair
-// from its mode rows, for receptors of any size).
-size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool rlds,
-                       unsigned* tabSlots);
+size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots);
 
+// `waves` waves per pose: 1 or 2.
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
-                        unsigned n, const unsigned* order, bool rlds, hipStream_t s);
-
-// The receptor is staged in LDS when k_modes then needs at most this much
-// (two workgroups per CU or more).
-constexpr size_t kModesStageLds = 64 * 1024;
+                        unsigned n, const unsigned* order, int waves, hipStream_t s);
 
 hipError_t launch_debug_rigid(const DProt& rec, const DProt& lig, const DGrid& inner,
                               const DGrid& outer, const DNL& nl, const DPar& par,

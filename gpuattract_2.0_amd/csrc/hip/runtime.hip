@@ -560,18 +560,11 @@ att_status att_score_rigid(att_ctx* c, att_handle hr, att_handle hl, att_handle 
 }
 
 static att_status check_modes(att_ctx* c, ProtObj* rec, ProtObj* lig, GridObj* gr, GridObj* gl,
-                              ParObj* p, bool* rlds) {
+                              ParObj* p) {
     att_status st;
     if ((st = check_pair(c, rec, lig, gr, p)) != ATT_OK) return st;
     if (rec->maxMType >= gl->inner.ngrids || rec->maxMType >= gl->outer.ngrids)
         return fail(c, ATT_ERR_INVALID, "receptor mapped type has no ligand grid");
-    if (lig->d.n > 0xFFFFu) return fail(c, ATT_ERR_UNSUPPORTED, "ligand exceeds 65535 beads");
-    // static LDS: the receptor mode-gradient rows of the four waves
-    const size_t stat = sizeof(float) * 4 * ATT_MODES_MAX + 16;
-    *rlds = modes_lds_bytes(rec->d, lig->d, p->d, true, nullptr) + stat <=
-            std::min(kModesStageLds, c->maxLdsPerBlock);
-    if (modes_lds_bytes(rec->d, lig->d, p->d, *rlds, nullptr) + stat > c->maxLdsPerBlock)
-        return fail(c, ATT_ERR_UNSUPPORTED, "pair table too large for the modes kernel's LDS");
     return ATT_OK;
 }
 
@@ -590,15 +583,15 @@ att_status att_score_modes_dev(att_ctx* c, att_handle hr, att_handle hl, att_han
     if ((st = resolve(c, hr, hl, hgr, hp, &rec, &lig, &gr, &p)) != ATT_OK) return st;
     GridObj* gl = lookup<GridObj>(c, hgl, Kind::Grid);
     if (!gl) return fail(c, ATT_ERR_HANDLE, "unknown ligand grid handle");
-    bool rlds = true;
-    if ((st = check_modes(c, rec, lig, gr, gl, p, &rlds)) != ATT_OK) return st;
+    if ((st = check_modes(c, rec, lig, gr, gl, p)) != ATT_OK) return st;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const unsigned* order = nullptr;
     if ((st = pose_order(c, reinterpret_cast<const float*>(d_dofs), 26, n, kForwardOrderMin, s, &order)) != ATT_OK)
         return st;
     ATT_HIP(c, launch_modes(rec->d, lig->d, gr->inner, gr->outer, gr->nl, gl->inner, gl->outer,
                             p->d, reinterpret_cast<const float*>(d_dofs),
-                            reinterpret_cast<float*>(d_out), (unsigned)n, order, rlds, s));
+                            reinterpret_cast<float*>(d_out), (unsigned)n, order,
+                            pair_waves(lig->d.n), s));
     if (order) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }

@@ -110,21 +110,6 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
 
 constexpr int kModesVals = 13 + 2 * ATT_MODES_MAX;
 
-// Beads per super-chunk of k_modes' ligand phase and their LDS (as k_rigid's,
-// per workgroup).
-constexpr unsigned kModesSuper = 256;
-constexpr size_t kModesSuperBytes =
-    kModesSuper * (16 + 8 + 2) + sizeof(unsigned) * queue_counters(kModesSuper);
-
-// Byte offset of the super-chunk arrays in k_modes' dynamic LDS: after the
-// pair table and, when the receptor is staged (rlds), its deformed
-// coordinates (xyz floats) and types (u16).
-__host__ __device__ inline size_t modes_super_offset(const DProt& rec, unsigned tabSlots, bool rlds) {
-    size_t q = sizeof(att_pair_params) * (size_t)tabSlots;
-    if (rlds) q += sizeof(float) * 3 * (size_t)rec.n + sizeof(unsigned short) * (((size_t)rec.n + 1) & ~(size_t)1);
-    return (q + 15) & ~(size_t)15;
-}
-
 // The deformed receptor position of bead j from its padded mode row (one or
 // two 16-byte loads), in the order of deform(): bit-identical coordinates.
 template <int MM>
@@ -159,42 +144,72 @@ struct RecPosDeform {
     }
 };
 
-// One 256-thread workgroup per pose.  Phase A: every receptor bead deformed
-// and moved into the ligand frame (invertDOF, rotate_translate_deform), scored
-// on the ligand grid, its force rotated back (rotate_forces) and projected on
-// the receptor modes; with rlds the deformed coordinates stay in LDS for the
-// pair loop, else the pair loop deforms its receptor beads again from their
-// mode rows (receptors of any size).  Phase B: the ligand in super-chunks of
-// kModesSuper beads, as k_rigid: grid term and list cell per bead (P1), the
-// class-ordered queue (P2), one bead per lane walking its list in the
-// reference's order with the receptor reaction projected on the receptor
-// modes (P3), and the reference's bead-order reductions by wave 0 (P4):
-// E, F and the torque tensor in lanes 0..12, reduceModeForce of the ligand
-// (R^T f . mode) in lanes 13..12 + M.  Semantics of
-// service/CPUEnergyService6DModes.tpp:121-327.
-template <int MM, bool RLDS>
-__global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter,
-                                               DNL nl, DGrid lInner, DGrid lOuter, DPar par,
-                                               const float* __restrict__ dofs,
-                                               float* __restrict__ res, unsigned n,
-                                               unsigned tabSlots,
-                                               const unsigned* __restrict__ order) {
-    extern __shared__ __align__(16) att_pair_params s_dyn[];
-    __shared__ float s_red[4][ATT_MODES_MAX];
-    const att_pair_params* tab = stage_table(par, s_dyn);
-    float* s_rx = reinterpret_cast<float*>(s_dyn + tabSlots);
-    float* s_ry = s_rx + rec.n;
-    float* s_rz = s_ry + rec.n;
-    unsigned short* s_rt = reinterpret_cast<unsigned short*>(s_rz + rec.n);
-    char* sb = reinterpret_cast<char*>(s_dyn) + modes_super_offset(rec, tabSlots, RLDS);
-    float4* s_f = reinterpret_cast<float4*>(sb);
-    uint2* s_n = reinterpret_cast<uint2*>(s_f + kModesSuper);
-    unsigned short* s_q = reinterpret_cast<unsigned short*>(s_n + kModesSuper);
-    unsigned* s_cc = reinterpret_cast<unsigned*>(s_q + kModesSuper);
+// reduceModeForce of the ligand (model/reduction_modes.h:28-57) for mode mm
+// over beads f[0..m) (LDS) of a super-chunk starting at bead sc, in bead
+// order: red -= (R^T f) . mode.  The mode vectors of eight beads are loaded
+// together.
+__device__ __forceinline__ float reduce_modes_ref(float red, const float* RT, const float4* f,
+                                                  const DProt& lig, unsigned mm, unsigned sc,
+                                                  unsigned m) {
+    const size_t nM = (size_t)lig.M * lig.n;
+    const float* mx = lig.modes + (size_t)mm * lig.n + sc;
+    unsigned k = 0;
+    for (; k + 8 <= m; k += 8) {
+        float ax[8], ay[8], az[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ax[u] = mx[k + u];
+            ay[u] = mx[nM + k + u];
+            az[u] = mx[2 * nM + k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float4 v = f[k + u];
+            const V3 fa = matvec(RT, v.x, v.y, v.z);
+            red -= fa.x * ax[u] + fa.y * ay[u] + fa.z * az[u];
+        }
+    }
+    for (; k < m; ++k) {
+        const float4 v = f[k];
+        const V3 fa = matvec(RT, v.x, v.y, v.z);
+        red -= fa.x * mx[k] + fa.y * mx[nM + k] + fa.z * mx[2 * nM + k];
+    }
+    return red;
+}
+
+// One workgroup of W waves per pose (W = 1 below 384 ligand beads, else 2).
+// Phase A: every receptor bead deformed and moved into the ligand frame
+// (invertDOF, rotate_translate_deform), scored on the ligand grid, its force
+// rotated back (rotate_forces) and projected on the receptor modes.  Phase B:
+// the ligand in super-chunks of 256 * W beads, as k_rigid: grid term and list
+// cell per bead (P1), the class-ordered queue (P2), one bead per lane walking
+// its list in the reference's order against the deformed receptor (each
+// receptor bead deformed again from its mode row, so receptors of any size
+// need no LDS) with the reaction projected on the receptor modes (P3), and
+// the reference's bead-order reductions (P4): E, F and the torque tensor in
+// lanes 0..12 of wave 0, the ligand's reduceModeForce in lanes of the last
+// wave.  Semantics of service/CPUEnergyService6DModes.tpp:121-327.
+// Occupancy: at least 4 waves per SIMD (3 with 10 modes), as the adjoint.
+template <int MM, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10 ? 3 : 4))) void k_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter,
+                                                  DNL nl, DGrid lInner, DGrid lOuter, DPar par,
+                                                  const float* __restrict__ dofs,
+                                                  float* __restrict__ res, unsigned n,
+                                                  const unsigned* __restrict__ order) {
+    constexpr unsigned kSuper = 256u * W;
+    extern __shared__ __align__(16) att_pair_params s_tab[];
+    __shared__ float4 s_f[kSuper];
+    __shared__ uint2 s_n[kSuper];
+    __shared__ unsigned short s_q[kSuper];
+    __shared__ unsigned s_cc[queue_counters(kSuper)];
+    __shared__ float s_red[W][ATT_MODES_MAX];
+    __shared__ float s_gl[ATT_MODES_MAX];
+    const att_pair_params* tab = stage_table(par, s_tab);  // visible after phase B's first barrier
     if (blockIdx.x >= n) return;  // uniform per workgroup
     const unsigned pose = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
-    const unsigned lane = threadIdx.x & (kWave - 1);
-    const unsigned wave = threadIdx.x / kWave;
+    const unsigned tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    constexpr unsigned nthr = 64u * W;
+    constexpr unsigned modeWave = W - 1;  // runs the ligand's reduceModeForce
     const float* d = dofs + 26 * (size_t)pose;
     const float tx = d[0], ty = d[1], tz = d[2];
     constexpr int MA = MM > 0 ? MM : 1;
@@ -218,15 +233,11 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
     for (int m = 0; m < MA; ++m) mR[m] = 0.f;
 
     // Phase A: receptor beads.
-    for (unsigned j = threadIdx.x; j < rec.n; j += blockDim.x) {
+    for (unsigned j = tid; j < rec.n; j += nthr) {
         const float4 x = rec.pos[j];
         BeadModes<MM> md;
         load_modes(rec, j, md);
         const V3 df = deform(md, rec.M, x.x, x.y, x.z, ampR);
-        if (RLDS) {
-            s_rx[j] = df.x; s_ry[j] = df.y; s_rz[j] = df.z;
-            s_rt[j] = (unsigned short)rec.type[j];
-        }
         V3 p = matvec(Rinv, df.x, df.y, df.z);
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
         const float4 pot = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
@@ -234,7 +245,6 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         load_modes(rec, j, md);  // reloaded (cache hit) rather than held across the lookup
         mode_dots(md, rec.M, f.x, f.y, f.z, mR);
     }
-    __syncthreads();
 
     // Phase B: ligand beads.
     const float4* __restrict__ rpos = rec.pos;
@@ -263,12 +273,11 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         return p;
     };
     const unsigned Ml = lig.M, nL = lig.n;
-    const size_t nML = (size_t)Ml * nL;
-    float red = 0.f;  // wave 0: the reference's reductions, one per lane
-    for (unsigned sc = 0; sc < nL; sc += kModesSuper) {
-        const unsigned m = min(kModesSuper, nL - sc);
+    float red = 0.f;  // wave 0: E, F, torque tensor; modeWave: ligand modes
+    for (unsigned sc = 0; sc < nL; sc += kSuper) {
+        const unsigned m = min(kSuper, nL - sc);
         // P1
-        for (unsigned k = threadIdx.x; k < m; k += blockDim.x) {
+        for (unsigned k = tid; k < m; k += nthr) {
             float4 x;
             const V3 p = place(sc + k, x);
             s_f[k] = pot_bead(rInner, rOuter, p.x, p.y, p.z, lig.mtype[sc + k], x.w, nullptr);
@@ -276,72 +285,55 @@ __global__ __launch_bounds__(256) void k_modes(DProt rec, DProt lig, DGrid rInne
         }
         __syncthreads();
         // P2
-        const unsigned qn = class_queue([&](unsigned k) { return s_n[k].x; }, m, s_q, s_cc,
-                                        threadIdx.x, blockDim.x, [] { __syncthreads(); });
+        const unsigned qn = class_queue([&](unsigned k) { return s_n[k].x; }, m, s_q, s_cc, tid,
+                                        nthr, [] { __syncthreads(); });
         // P3
-        for (unsigned q0 = 0; q0 < qn; q0 += blockDim.x) {
-            if (q0 + threadIdx.x < qn) {
-                const unsigned k = s_q[q0 + threadIdx.x];
+        for (unsigned q0 = 0; q0 < qn; q0 += nthr) {
+            if (q0 + tid < qn) {
+                const unsigned k = s_q[q0 + tid];
                 float4 x;
                 const V3 p = place(sc + k, x);
-                float4 a;
-                if (RLDS)
-                    a = nl_walk(nl, par, tab,
-                                [=](unsigned j) { return make_float3(s_rx[j], s_ry[j], s_rz[j]); },
-                                [=](unsigned j) { return (unsigned)s_rt[j]; }, RecChargeG{rpos},
-                                reaction, p.x, p.y, p.z, lig.type[sc + k], x.w, s_n[k], nullptr);
-                else
-                    a = nl_walk(nl, par, tab, RecPosDeform<MM>{rpos, rmodesT, rowT, Mr, ampR},
-                                RecTypeG{rec.type}, RecChargeG{rpos}, reaction, p.x, p.y, p.z,
-                                lig.type[sc + k], x.w, s_n[k], nullptr);
+                const float4 a = nl_walk<1>(nl, par, tab, RecPosDeform<MM>{rpos, rmodesT, rowT, Mr, ampR},
+                                         RecTypeG{rec.type}, RecChargeG{rpos}, reaction, p.x, p.y,
+                                         p.z, lig.type[sc + k], x.w, s_n[k], nullptr);
                 float4 f = s_f[k];
                 f.x = f.x + a.x; f.y = f.y + a.y; f.z = f.z + a.z; f.w = f.w + a.w;
                 s_f[k] = f;
             }
         }
         __syncthreads();
-        // P4 (wave 0): reducePotForce, reduceTorque and the ligand's
-        // reduceModeForce (model/reduction_modes.h:28-57), in bead order.
-        if (wave == 0) {
-            red = reduce_beads_ref(red, lane, s_f, lig.pos, sc, m);
-            if (lane >= 13u && lane < 13u + Ml) {
-                const unsigned mm = lane - 13u;
-                const float* mx = lig.modes + (size_t)mm * nL + sc;
-                for (unsigned k = 0; k < m; ++k) {
-                    const float4 f = s_f[k];
-                    const V3 fa = matvec(RT, f.x, f.y, f.z);
-                    red -= fa.x * mx[k] + fa.y * mx[nML + k] + fa.z * mx[2 * nML + k];
-                }
-            }
-        }
+        // P4: reducePotForce, reduceTorque and the ligand's reduceModeForce, in
+        // bead order.
+        if (wave == 0) red = reduce_beads_ref(red, lane, s_f, lig.pos, sc, m);
+        const unsigned ml = W > 1 ? lane : lane - 13u;  // mode of this lane in modeWave
+        if (wave == modeWave && ml < Ml) red = reduce_modes_ref(red, RT, s_f, lig, ml, sc, m);
         __syncthreads();  // the super-chunk's arrays are reused
     }
 
-    // Receptor mode gradients: wave butterflies, then the four waves in order.
+    // Receptor mode gradients: wave butterflies, then the waves in order.
 #pragma unroll
     for (int m = 0; m < MM; ++m) mR[m] = wave_sum(mR[m]);
     if (lane == 0) {
 #pragma unroll
         for (int m = 0; m < ATT_MODES_MAX; ++m) s_red[wave][m] = m < MM ? mR[m < MM ? m : 0] : 0.f;
     }
+    if (wave == modeWave) {
+        const unsigned ml = W > 1 ? lane : lane - 13u;
+        if (ml < ATT_MODES_MAX) s_gl[ml] = red;
+    }
     __syncthreads();
     if (wave == 0) {
         float o[7];
         finish_ref(red, t, o);
-        float gl[ATT_MODES_MAX];
-#pragma unroll
-        for (int m = 0; m < ATT_MODES_MAX; ++m)
-            gl[m] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(red), 13 + m));
         if (lane == 0) {
             float* r = res + 27 * (size_t)pose;
 #pragma unroll
             for (int q = 0; q < 7; ++q) r[q] = o[q];
-            const int nw = blockDim.x / kWave;
             for (unsigned m = 0; m < ATT_MODES_MAX; ++m) {
                 float s = s_red[0][m];
-                for (int w = 1; w < nw; ++w) s += s_red[w][m];
+                for (int w = 1; w < W; ++w) s += s_red[w][m];
                 r[7 + m] = m < rec.M ? correct_mode(s, rec.modeForce[m]) : 0.f;
-                r[17 + m] = m < lig.M ? correct_mode(gl[m], lig.modeForce[m]) : 0.f;
+                r[17 + m] = m < lig.M ? correct_mode(s_gl[m], lig.modeForce[m]) : 0.f;
             }
         }
     }
@@ -521,30 +513,28 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
     return hipGetLastError();
 }
 
-// Dynamic LDS of k_modes: pair table, with rlds the deformed receptor (xyz
-// floats, u16 types), then the ligand super-chunk arrays.
-size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bool rlds,
-                       unsigned* tabSlots) {
+// Dynamic LDS of k_modes: the pair table (the super-chunk arrays are static).
+size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots) {
+    (void)rec;
     (void)lig;
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     if (tabSlots) *tabSlots = slots;
-    return modes_super_offset(rec, slots, rlds) + kModesSuperBytes;
+    return sizeof(att_pair_params) * (size_t)slots;
 }
 
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
-                        unsigned n, const unsigned* order, bool rlds, hipStream_t s) {
+                        unsigned n, const unsigned* order, int waves, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    unsigned slots = 0;
-    const size_t lds = modes_lds_bytes(rec, lig, par, rlds, &slots);
+    const size_t lds = modes_lds_bytes(rec, lig, par, nullptr);
     const unsigned mm = rec.M > lig.M ? rec.M : lig.M;
-#define ATT_LAUNCH_MODES(MMV, RL)                                                               \
-    hipLaunchKernelGGL((k_modes<MMV, RL>), dim3(n), dim3(256), lds, s, rec, lig, rInner, rOuter, \
-                       nl, lInner, lOuter, par, dofs, res, n, slots, order)
-    if (mm == 0) { if (rlds) ATT_LAUNCH_MODES(0, true); else ATT_LAUNCH_MODES(0, false); }
-    else if (mm <= 5) { if (rlds) ATT_LAUNCH_MODES(5, true); else ATT_LAUNCH_MODES(5, false); }
-    else { if (rlds) ATT_LAUNCH_MODES(ATT_MODES_MAX, true); else ATT_LAUNCH_MODES(ATT_MODES_MAX, false); }
+#define ATT_LAUNCH_MODES(MMV, WV)                                                                 \
+    hipLaunchKernelGGL((k_modes<MMV, WV>), dim3(n), dim3(64 * WV), lds, s, rec, lig, rInner, rOuter, \
+                       nl, lInner, lOuter, par, dofs, res, n, order)
+    if (mm == 0) { if (waves > 1) ATT_LAUNCH_MODES(0, 2); else ATT_LAUNCH_MODES(0, 1); }
+    else if (mm <= 5) { if (waves > 1) ATT_LAUNCH_MODES(5, 2); else ATT_LAUNCH_MODES(5, 1); }
+    else { if (waves > 1) ATT_LAUNCH_MODES(ATT_MODES_MAX, 2); else ATT_LAUNCH_MODES(ATT_MODES_MAX, 1); }
 #undef ATT_LAUNCH_MODES
     return hipGetLastError();
 }
