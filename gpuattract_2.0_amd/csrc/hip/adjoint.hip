@@ -273,7 +273,7 @@ struct PoseLDS {
 };
 
 // Adjoint of one pose by one workgroup of WAVES waves, the ligand in
-// super-chunks of PoseLDS<WAVES>::kSuper beads:
+// super-chunks of equal size, at most PoseLDS<WAVES>::kSuper beads (super_chunk):
 //  P1  every bead: transform, grid term with its exact gradient (and the
 //      basis-grid parameter derivatives), neighbour-list cell;
 //  P2  beads with a list, longest lists first (class_queue);
@@ -329,9 +329,9 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
         return p;
     };
 
-    constexpr unsigned kSuper = PoseLDS<WAVES>::kSuper;
-    for (unsigned sc = 0; sc < lig.n; sc += kSuper) {
-        const unsigned m = min(kSuper, lig.n - sc);
+    const unsigned span = super_chunk(lig.n, PoseLDS<WAVES>::kSuper);
+    for (unsigned sc = 0; sc < lig.n; sc += span) {
+        const unsigned m = min(span, lig.n - sc);
         // P1
         for (unsigned k = tid; k < m; k += nthr) {
             float4 x;
@@ -364,8 +364,17 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
         }
         __syncthreads();
         // P4
-        if (tid == eThread)
-            for (unsigned k = 0; k < m; ++k) esum += S.g[k].w;
+        if (tid == eThread) {
+            unsigned k = 0;
+            for (; k + 8 <= m; k += 8) {  // eight loads in flight, the sum sequential
+                float e8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) e8[u] = S.g[k + u].w;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) esum += e8[u];
+            }
+            for (; k < m; ++k) esum += S.g[k].w;
+        }
         if (wave == 0) {
             for (unsigned k = lane; k < m; k += kWave) {
                 float4 x;

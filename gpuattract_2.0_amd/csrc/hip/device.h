@@ -629,24 +629,34 @@ __device__ __forceinline__ float reduce_beads_ref(float red, unsigned lane, cons
                                                   const float4* pos, unsigned first, unsigned m) {
     if (lane >= 13u) return red;
     const unsigned fc = lane == 0 ? 3u : lane < 4u ? lane - 1u : (lane - 4u) / 3u;
-    const unsigned xc = lane < 4u ? 0u : (lane - 4u) % 3u;
-    const bool prod = lane >= 4u;
+    const unsigned xc = lane < 4u ? 3u : (lane - 4u) % 3u;  // 3: no coordinate factor
     const float* ff = reinterpret_cast<const float*>(f);
-    const float* pf = reinterpret_cast<const float*>(pos + first);
+    const float4* px = pos + first;
+    auto coord = [xc](const float4& x) { return xc == 0u ? x.x : xc == 1u ? x.y : xc == 2u ? x.z : 1.f; };
     unsigned k = 0;
-    // the loads of four beads are issued together; the sum stays sequential
-    for (; k + 4 <= m; k += 4) {
-        float fv[4], xv[4];
+    // eight beads' loads are issued together (the coordinates are the same for
+    // every lane: scalar loads); the sum stays sequential
+    for (; k + 8 <= m; k += 8) {
+        float4 xs[8];
+        float fv[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
+            xs[u] = px[k + u];
             fv[u] = ff[4 * (k + u) + fc];
-            xv[u] = prod ? pf[4 * (k + u) + xc] : 1.f;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) red += xv[u] * fv[u];  // 1 * f = f exactly
+        for (int u = 0; u < 8; ++u) red += coord(xs[u]) * fv[u];  // 1 * f = f exactly
     }
-    for (; k < m; ++k) red += (prod ? pf[4 * k + xc] : 1.f) * ff[4 * k + fc];
+    for (; k < m; ++k) red += coord(px[k]) * ff[4 * k + fc];
     return red;
+}
+
+// Beads per super-chunk for n beads and at most smax (a multiple of 64) per
+// super-chunk: the fewest super-chunks, of equal size rounded up to 64.
+__host__ __device__ inline unsigned super_chunk(unsigned n, unsigned smax) {
+    const unsigned parts = n ? (n + smax - 1) / smax : 1u;
+    const unsigned per = (n + parts - 1) / parts;
+    return (per + 63u) & ~63u;
 }
 
 // The epilogue of those reductions, by every lane of the wave that ran them:

@@ -65,8 +65,9 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
         return p;
     };
     float red = 0.f;  // the reference's reduction held by this lane (reduce_beads_ref)
-    for (unsigned sc = 0; sc < lig.n; sc += kRigidSuper) {
-        const unsigned m = min(kRigidSuper, lig.n - sc);
+    const unsigned span = super_chunk(lig.n, kRigidSuper);
+    for (unsigned sc = 0; sc < lig.n; sc += span) {
+        const unsigned m = min(span, lig.n - sc);
         // P1: one bead per lane: rotate_translate, potForce, the list's cell.
         for (unsigned k = lane; k < m; k += kWave) {
             float4 x;
@@ -181,7 +182,7 @@ __device__ __forceinline__ float reduce_modes_ref(float red, const float* RT, co
 // Phase A: every receptor bead deformed and moved into the ligand frame
 // (invertDOF, rotate_translate_deform), scored on the ligand grid, its force
 // rotated back (rotate_forces) and projected on the receptor modes.  Phase B:
-// the ligand in super-chunks of 256 * W beads, as k_rigid: grid term and list
+// the ligand in super-chunks of at most 256 * W beads, as k_rigid: grid term and list
 // cell per bead (P1), the class-ordered queue (P2), one bead per lane walking
 // its list in the reference's order against the deformed receptor (each
 // receptor bead deformed again from its mode row, so receptors of any size
@@ -274,8 +275,9 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
     };
     const unsigned Ml = lig.M, nL = lig.n;
     float red = 0.f;  // wave 0: E, F, torque tensor; modeWave: ligand modes
-    for (unsigned sc = 0; sc < nL; sc += kSuper) {
-        const unsigned m = min(kSuper, nL - sc);
+    const unsigned span = super_chunk(nL, kSuper);
+    for (unsigned sc = 0; sc < nL; sc += span) {
+        const unsigned m = min(span, nL - sc);
         // P1
         for (unsigned k = tid; k < m; k += nthr) {
             float4 x;
