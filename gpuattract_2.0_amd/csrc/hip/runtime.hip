@@ -1252,6 +1252,18 @@ att_status att_batch_bucket_info(att_ctx* c, const att_batch* b, unsigned bucket
     return ATT_OK;
 }
 
+att_status att_batch_pair_bucket(att_ctx* c, const att_batch* b, unsigned pair, unsigned* bucket) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!b || !bucket || pair >= b->pairs.size()) return fail(c, ATT_ERR_INVALID, "bad pair");
+    for (size_t k = 0; k < b->buckets.size(); ++k)
+        for (unsigned q : b->buckets[k].pairs)
+            if (q == pair) {
+                *bucket = (unsigned)k;
+                return ATT_OK;
+            }
+    return fail(c, ATT_ERR_INVALID, "pair in no bucket");
+}
+
 att_status att_batch_set_shape(att_ctx* c, att_batch* b, unsigned bucket, int wavesPerPose,
                                unsigned posesPerTile) {
     if (!c) return ATT_ERR_INVALID;

@@ -178,6 +178,7 @@ def lib():
         L.att_batch_buckets.argtypes = [P]
         L.att_batch_buckets.restype = C.c_uint
         L.att_batch_bucket_info.argtypes = [P, P, C.c_uint, C.POINTER(BatchBucket)]
+        L.att_batch_pair_bucket.argtypes = [P, P, C.c_uint, C.POINTER(C.c_uint)]
         L.att_batch_set_shape.argtypes = [P, P, C.c_uint, C.c_int, C.c_uint]
         L.att_batch_tune.argtypes = [P, P, P]
         L.att_batch_run.argtypes = [P, P, P]
@@ -228,6 +229,11 @@ class Batch:
             self.ctx._chk(self.ctx.L.att_batch_bucket_info(self.ctx.h, self.h, b, C.byref(info)))
             out.append({f: getattr(info, f) for f, _ in BatchBucket._fields_})
         return out
+
+    def pair_bucket(self, pair):
+        k = C.c_uint()
+        self.ctx._chk(self.ctx.L.att_batch_pair_bucket(self.ctx.h, self.h, pair, C.byref(k)))
+        return k.value
 
     def set_shape(self, bucket, waves, tile):
         self.ctx._chk(self.ctx.L.att_batch_set_shape(self.ctx.h, self.h, bucket, waves, tile))

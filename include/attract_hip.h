@@ -318,6 +318,8 @@ att_status att_batch_create(att_ctx* ctx, const att_batch_pair* pairs, unsigned 
 att_status att_batch_destroy(att_ctx* ctx, att_batch* b);
 unsigned att_batch_buckets(const att_batch* b);
 att_status att_batch_bucket_info(att_ctx* ctx, const att_batch* b, unsigned bucket, att_batch_bucket* info);
+/* The bucket that scores pair `pair` (its index in att_batch_create's list). */
+att_status att_batch_pair_bucket(att_ctx* ctx, const att_batch* b, unsigned pair, unsigned* bucket);
 att_status att_batch_set_shape(att_ctx* ctx, att_batch* b, unsigned bucket, int wavesPerPose,
                                unsigned posesPerTile);
 att_status att_batch_tune(att_ctx* ctx, att_batch* b, void* stream);

@@ -35,6 +35,7 @@
 #undef SFX
 #undef RCOS
 #undef RSIN
+#include "oracle_kadj.inc"
 
 #define REAL double
 #define SFX _f64
@@ -65,6 +66,22 @@ int orc_adjoint_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* i
     return 0;
 }
 
+/* The FP32 adjoint in the HIP kernel's summation order (oracle_kadj.inc) for
+ * `waves` waves per pose; w (float weights) may be NULL. */
+int orc_adjoint_kernel_order(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                             const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                             const float* dof, size_t n, const float* w, int waves, double* out) {
+    if (waves != 1 && waves != 2 && waves != 4) return -1;
+    ctx_f32 x;
+    memset(&x, 0, sizeof(x));
+    x.rec = rec; x.lig = lig; x.par = par;
+    x.inner = inner; x.outer = outer; x.nl = nl;
+    ctx_init_f32(&x);
+    for (size_t p = 0; p < n; ++p) kadj_pose(&x, dof + 26 * p, w ? w[p] : 1.f, waves, out + 27 * p);
+    ctx_free_f32(&x);
+    return 0;
+}
+
 /* Pose minimization with VA13 (oracle_va13.c) on the FP32 adjoint: the state
  * is (angles, position, receptor modes, ligand modes) in double
  * (apps/em/meta.h:90-170); each request is rounded to float DOFs, as the
@@ -73,6 +90,7 @@ int orc_adjoint_f64(const orc_body* rec, const orc_body* lig, const orc_igrid* i
 typedef struct {
     const ctx_f32* x32;
     const ctx_f64* x64;
+    int waves;  /* > 0: the FP32 adjoint in the kernel's order for this many waves */
     unsigned Mr, Ml;
     float dof[26];
     double res[27];
@@ -87,7 +105,8 @@ static void min_fg(void* p, const double* s, double* f, double* g) {
     }
     for (unsigned k = 0; k < m->Mr; ++k) m->dof[6 + k] = (float)s[6 + k];
     for (unsigned k = 0; k < m->Ml; ++k) m->dof[16 + k] = (float)s[6 + m->Mr + k];
-    if (m->x32) adj_pose_f32(m->x32, 0, 0, m->dof, 1.0, m->res, 0, 0);
+    if (m->x32 && m->waves > 0) kadj_pose(m->x32, m->dof, 1.f, m->waves, m->res);
+    else if (m->x32) adj_pose_f32(m->x32, 0, 0, m->dof, 1.0, m->res, 0, 0);
     else adj_pose_f64(m->x64, 0, 0, m->dof, 1.0, m->res, 0, 0);
     *f = m->res[0];
     for (int k = 0; k < 3; ++k) {
@@ -139,6 +158,29 @@ int orc_minimize_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* 
     min_ctx m;
     memset(&m, 0, sizeof(m));
     m.x32 = &x;
+    m.Mr = rec->M;
+    m.Ml = lig->M;
+    minimize_poses(&m, dof, n, ivmax, dof_out, e_last, e_best, nfun);
+    ctx_free_f32(&x);
+    return 0;
+}
+
+/* The same on the kernel-order FP32 adjoint for `waves` waves per pose: the
+ * trajectories of the device minimizer, bit for bit. */
+int orc_minimize_kernel_order(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                              const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                              const float* dof, size_t n, int ivmax, int waves, float* dof_out,
+                              double* e_last, double* e_best, int* nfun) {
+    if (waves != 1 && waves != 2 && waves != 4) return -1;
+    ctx_f32 x;
+    memset(&x, 0, sizeof(x));
+    x.rec = rec; x.lig = lig; x.par = par;
+    x.inner = inner; x.outer = outer; x.nl = nl;
+    ctx_init_f32(&x);
+    min_ctx m;
+    memset(&m, 0, sizeof(m));
+    m.x32 = &x;
+    m.waves = waves;
     m.Mr = rec->M;
     m.Ml = lig->M;
     minimize_poses(&m, dof, n, ivmax, dof_out, e_last, e_best, nfun);

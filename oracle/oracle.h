@@ -142,6 +142,17 @@ void orc_va13_update(double* a, int n, double* z, double sig, double* w, int* ir
 /* Minimizes n poses (26-float DOF_6D_Modes layout in and out) with VA13 on the
  * FP32 exact adjoint; the state is (angles, position, receptor modes, ligand
  * modes). */
+/* The FP32 adjoint evaluated in the HIP kernel's summation order for `waves`
+ * (1, 2 or 4) waves per pose; dof = 26 floats per pose, w = float weights or
+ * NULL, out = 27 per pose (no parameter gradient).  -1 for other waves. */
+int orc_adjoint_kernel_order(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                             const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                             const float* dof, size_t n, const float* w, int waves, double* out);
+/* VA13 on that adjoint (the device minimizer's trajectories). */
+int orc_minimize_kernel_order(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
+                              const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
+                              const float* dof, size_t n, int ivmax, int waves, float* dof_out,
+                              double* e_last, double* e_best, int* nfun);
 int orc_minimize_f32(const orc_body* rec, const orc_body* lig, const orc_igrid* inner,
                      const orc_igrid* outer, const orc_nlgrid* nl, const orc_params* par,
                      const float* dof, size_t n, int ivmax, float* dof_out, double* e_last,

@@ -317,6 +317,76 @@ def minimize(wl, dofs=None, ivmax=500, f64=False):
     return out, el, eb, nf
 
 
+def kernel_waves(wl):
+    """Waves per pose the HIP library uses for a single pair (1 below 384
+    ligand beads, else 2)."""
+    return 1 if wl.lig.n < 384 else 2
+
+
+def _chunks(n, parts):
+    step = max(1, -(-n // parts))
+    return [(a, min(n, a + step)) for a in range(0, n, step)]
+
+
+def adjoint_kernel_order(wl, dofs=None, w=None, waves=None, nthreads=8):
+    """The FP32 exact adjoint in the HIP kernel's summation order for `waves`
+    waves per pose (default: the single-pair choice): (n, 27) float64 holding
+    float32 values; equal to the kernel's output bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+    L = oracle_lib()
+    L.orc_adjoint_kernel_order.restype = C.c_int
+    L.orc_adjoint_kernel_order.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                           C.POINTER(IGridC), C.POINTER(NLGridC),
+                                           C.POINTER(OrcParamsC), P, C.c_size_t, P, C.c_int, P]
+    d = as_modes_dofs(wl, dofs)
+    n = d.shape[0]
+    out = np.zeros((n, 27), np.float64)
+    ww = None if w is None else np.ascontiguousarray(w, np.float32)
+    W = kernel_waves(wl) if waves is None else waves
+    args = (body_c(wl.rec), body_c(wl.lig), igrid_c(wl.rec_inner), igrid_c(wl.rec_outer),
+            nl_c(wl.rec_nl), params_c(wl.params))
+
+    def run(ab):
+        a, b = ab
+        if b > a:
+            rc = L.orc_adjoint_kernel_order(*[C.byref(x) for x in args], _p(d[a:b]), b - a,
+                                            None if ww is None else _p(ww[a:b]), W, _p(out[a:b]))
+            assert rc == 0
+    with ThreadPoolExecutor(nthreads) as ex:
+        list(ex.map(run, _chunks(n, nthreads)))
+    return out
+
+
+def minimize_kernel_order(wl, dofs=None, ivmax=500, waves=None, nthreads=8):
+    """VA13 on the kernel-order FP32 adjoint (the device minimizer's
+    trajectories): (dofs_last (n, 26) float32, e_last, e_best, nfun)."""
+    from concurrent.futures import ThreadPoolExecutor
+    L = oracle_lib()
+    L.orc_minimize_kernel_order.restype = C.c_int
+    L.orc_minimize_kernel_order.argtypes = [C.POINTER(BodyC), C.POINTER(BodyC), C.POINTER(IGridC),
+                                            C.POINTER(IGridC), C.POINTER(NLGridC),
+                                            C.POINTER(OrcParamsC), P, C.c_size_t, C.c_int, C.c_int,
+                                            P, P, P, P]
+    d = as_modes_dofs(wl, dofs)
+    n = d.shape[0]
+    out = np.zeros((n, 26), np.float32)
+    el, eb = np.zeros(n), np.zeros(n)
+    nf = np.zeros(n, np.int32)
+    W = kernel_waves(wl) if waves is None else waves
+    args = (body_c(wl.rec), body_c(wl.lig), igrid_c(wl.rec_inner), igrid_c(wl.rec_outer),
+            nl_c(wl.rec_nl), params_c(wl.params))
+
+    def run(ab):
+        a, b = ab
+        if b > a:
+            rc = L.orc_minimize_kernel_order(*[C.byref(x) for x in args], _p(d[a:b]), b - a, ivmax, W,
+                                             _p(out[a:b]), _p(el[a:b]), _p(eb[a:b]), _p(nf[a:b]))
+            assert rc == 0
+    with ThreadPoolExecutor(nthreads) as ex:
+        list(ex.map(run, _chunks(n, nthreads)))
+    return out, el, eb, nf
+
+
 def beads(wl, dof):
     """Per-bead diagnostics of one pose: transformed coordinates, forces/energy,
     cell indices (inner xyz, outer xyz, NL xyz; -1 unused) and NL pair counts."""

@@ -1,4 +1,8 @@
-"""HIP adjoint vs the oracle's FP64 adjoint (needs a GPU).
+"""HIP adjoint vs the oracle's adjoint (needs a GPU).
+
+Bit for bit: every output but the parameter gradient against the oracle's
+FP32 adjoint in the kernel's summation order; the energy against the forward
+path (the reference's energy).
 
 Tolerance: per-pose gradients within 1e-5 of the condition scale (sum of
 |terms|, oracle FP64) against the oracle's FP32 adjoint, which forms the bead
@@ -60,10 +64,27 @@ def test_adjoint_against_oracle(ctx, O, which, wl_adj_rigid, wl_adj_modes):
     assert e_gpu.max() <= 1.1 * e_ref.max() + 1e-6
     scale = np.abs(pref).max()
     assert np.abs(pg - pref).max() <= 1e-4 * scale
-    # the adjoint's energy is the forward's energy
+    # the adjoint's energy is the forward's energy, bit for bit
     fwd = ctx.score_modes(hs, wl.dofs) if wl.modes else ctx.score_rigid(hs, wl.dofs)
-    o64, s64 = (O.score_modes if wl.modes else O.score_rigid)(wl, f64=True)
-    assert cond_err(g[:, 0], fwd[:, 0], s64[:, 0]).max() <= TOL
+    assert np.array_equal(g[:, 0], fwd[:, 0])
+
+
+@pytest.mark.parametrize("which", ["rigid", "modes", "rigid500", "modes500"])
+def test_adjoint_bit_exact_in_kernel_order(ctx, W, O, which, wl_adj_rigid, wl_adj_modes):
+    """Every output but the parameter gradient equals the oracle's FP32
+    adjoint evaluated in the kernel's summation order (one wave per pose under
+    384 ligand beads, two above), weights included; the energy is the
+    reference's (bead-order sums of list-order folds)."""
+    wl = {"rigid": wl_adj_rigid, "modes": wl_adj_modes,
+          "rigid500": W.make_workload("adj rigid 500", 55, 700, 500, 150, with_basis=True),
+          "modes500": W.make_workload("adj modes 500", 56, 700, 500, 150, 5, 5, with_basis=True)}[which]
+    hs = _upload(ctx, wl)
+    w = np.random.default_rng(4).uniform(-1.5, 2.0, wl.dofs.shape[0]).astype(np.float32)
+    g = ctx.adjoint(hs, wl.dofs, w=w)
+    k = O.adjoint_kernel_order(wl, w=w)[:, :g.shape[1]].astype(np.float32)
+    assert np.array_equal(g.view(np.uint32), k.view(np.uint32))
+    fwd = ctx.score_modes(hs, wl.dofs) if wl.modes else ctx.score_rigid(hs, wl.dofs)
+    assert np.array_equal(g[:, 0], fwd[:, 0])
 
 
 def test_adjoint_without_parameters_and_device_path(ctx, wl_adj_modes):

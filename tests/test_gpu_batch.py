@@ -4,8 +4,9 @@
   beads, 2 above) a batch returns, pair by pair, exactly the single-pair
   adjoint (same device code, same order).
 - Every launch shape (1, 2 or 4 waves per pose, any tile size) and the shape
-  the autotuner picks agree with the oracle's FP32 adjoint within 1e-5 of the
-  condition scale, like the single-pair adjoint.
+  the autotuner picks return, pair by pair, the oracle's FP32 adjoint in the
+  kernel's summation order for the bucket's waves per pose, bit for bit, and
+  the oracle's sequential FP32 adjoint within 1e-5 of the condition scale.
 - Replays of the captured graph are identical, with eager work between them.
 """
 import numpy as np
@@ -79,14 +80,17 @@ def test_batch_equals_single_pair_at_same_shape(ctx, staged):
     b.close()
 
 
-def _check_oracle(O, staged, got):
-    for s, g in zip(staged, got):
+def _check_oracle(O, staged, got, waves):
+    """waves[k]: the waves per pose pair k was scored with."""
+    for s, g, wv in zip(staged, got, waves):
         wl = s["wl"]
         ref, sc = O.adjoint(wl, scale=True)
         r32 = O.adjoint(wl, f32=True)
         cols = [c for c in range(s["ow"]) if sc[:, c].max() > 0]
         e = cond_err(g[:, cols], r32[:, cols], sc[:, cols])
         assert e.max() <= TOL, (wl.name, e.max(axis=0))
+        k = O.adjoint_kernel_order(wl, waves=wv)[:, :s["ow"]].astype(np.float32)
+        assert np.array_equal(g.view(np.uint32), k.view(np.uint32)), (wl.name, wv)
 
 
 @pytest.mark.parametrize("waves,tile", [(1, 4), (2, 64), (4, 1)])
@@ -96,7 +100,7 @@ def test_batch_shapes_match_oracle(ctx, O, staged, waves, tile):
         b.set_shape(k, waves, tile)
     b.run()
     ctx.sync()
-    _check_oracle(O, staged, _results(ctx, staged))
+    _check_oracle(O, staged, _results(ctx, staged), [waves] * len(staged))
     b.close()
 
 
@@ -110,7 +114,7 @@ def test_tuned_batch_and_graph_replays(ctx, O, staged):
     b.run()
     ctx.sync()
     first = _results(ctx, staged)
-    _check_oracle(O, staged, first)
+    _check_oracle(O, staged, first, [info[b.pair_bucket(k)]["wavesPerPose"] for k in range(len(staged))])
     for _ in range(3):
         _single(ctx, staged[0])  # eager work between replays
         b.run()

@@ -1,17 +1,14 @@
 """Device-resident VA13 minimization (needs a GPU).
 
 The state machine is pinned to the oracle's VA13 bit for bit on the CPU
-(test_va13_state_machine.py); here the whole loop runs on the device with the
-GPU adjoint.  GPU and oracle scores differ in the last bits (summation order);
-on a rugged landscape that moves line-search decisions and, over many
-evaluations, sends trajectories to different minima.  The oracle shows the same
-sensitivity against itself (FP32 vs FP64 scoring), so it is the yardstick:
-  - after 10 evaluations, GPU and FP32 oracle final energies agree (1e-3
-    relative) at least as often as FP32 and FP64 oracle runs do, minus 0.1;
-  - after 80, likewise minus 0.15, with no systematic bias of the GPU's final
-    energies (median difference within the spread);
-  - no pose ends above its start, the reported DOFs reproduce the reported
-    energy exactly, every pose stays within its evaluation limit."""
+(test_va13_state_machine.py), and the adjoint it calls equals the oracle's
+FP32 adjoint in the kernel's summation order bit for bit
+(test_gpu_adjoint.py).  So the device minimizer must follow the oracle's
+trajectories exactly: for every pose the same final DOFs, last and best
+energies and evaluation counts as the oracle's VA13 on that adjoint
+(orc_minimize_kernel_order), after 10, 80 and 500 evaluations.
+Also: no pose ends above its start, the reported DOFs reproduce the reported
+energy exactly, every pose stays within its evaluation limit."""
 import numpy as np
 import pytest
 
@@ -34,22 +31,20 @@ def test_minimize_against_oracle(ctx, W, O, which):
         wl = W.make_workload("min modes", 82, 300, 80, 40, 3, 2, with_lig_grid=False)
     hs = ctx.upload_workload(wl)
     e0 = ctx.adjoint(hs, wl.dofs)[:, 0]
-    for ivmax, slack in ((10, 0.1), (80, 0.15)):
+    for ivmax in (10, 80, 500):
         d, el, eb, nf = ctx.minimize(hs, wl.dofs, ivmax)
-        _, _, oeb, onf = O.minimize(wl, ivmax=ivmax)
-        _, _, deb, _ = O.minimize(wl, ivmax=ivmax, f64=True)
-        assert np.all(nf <= ivmax) and np.all(onf <= ivmax)
+        od, oel, oeb, onf = O.minimize_kernel_order(wl, ivmax=ivmax)
+        od = np.ascontiguousarray(od[:, :d.shape[1]])  # rigid: 6 DOFs per pose
+        assert np.all(nf <= ivmax)
+        assert np.array_equal(nf, onf)
+        assert np.array_equal(d.view(np.uint32), od.view(np.uint32))
+        assert np.array_equal(el.astype(np.float64), oel)
+        assert np.array_equal(eb.astype(np.float64), oeb)
         assert np.all(eb <= e0 + 1e-6 * np.abs(e0))
         # the reported DOFs reproduce the reported energy
         assert np.array_equal(ctx.adjoint(hs, d)[:, 0], el)
-        agree = lambda a, b: np.mean(np.abs(a - b) / (np.abs(b) + 1.0) <= 1e-3)
-        gpu, self_ = agree(eb.astype(np.float64), oeb), agree(deb, oeb)
-        diff = eb.astype(np.float64) - oeb
-        print(f"{which}, {ivmax} evaluations: GPU vs FP32 oracle agree {gpu:.2f}, FP64 vs FP32 "
-              f"oracle {self_:.2f}; median E difference {np.median(diff):.3g}; mean evaluations "
-              f"{nf.mean():.1f} (oracle {onf.mean():.1f})")
-        assert gpu >= self_ - slack
-        assert abs(np.median(diff)) <= np.std(oeb) + 1e-6
+        print(f"{which}, {ivmax} evaluations: {len(nf)} poses equal to the oracle; mean "
+              f"evaluations {nf.mean():.1f}")
     assert np.mean(eb < e0 - 1e-3) > 0.8
 
 

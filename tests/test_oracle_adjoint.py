@@ -161,3 +161,25 @@ def test_parameter_gradient_follows_the_bead_type_not_the_grid_slot(W, O, wl_pgr
     a1, p1 = O.adjoint(wp, pgrad=True)
     assert np.array_equal(a0, a1) and np.array_equal(p0, p1)
     assert np.array_equal(O.score_rigid(wl), O.score_rigid(wp))
+
+
+@pytest.mark.parametrize("modes", [False, True])
+def test_kernel_order_adjoint(W, O, modes):
+    """The FP32 adjoint in the HIP kernel's summation order: its energy is the
+    reference's energy bit for bit (list-order folds per bead, bead-order sum:
+    the forward restatement, which equals the compiled reference on the golden
+    fixtures); its gradient is as close to FP64 as the sequential FP32
+    restatement's; only the receptor mode sums depend on the waves per pose."""
+    wl = W.make_workload("kadj", 71, 600, 450, 24, *((4, 3) if modes else ()), with_basis=True)
+    ref64, sc = O.adjoint(wl, scale=True)
+    r32 = O.adjoint(wl, f32=True)
+    fwd = (O.score_modes if modes else O.score_rigid)(wl)
+    ks = {wv: O.adjoint_kernel_order(wl, waves=wv) for wv in (1, 2, 4)}
+    for wv, k in ks.items():
+        assert np.array_equal(k[:, 0].astype(np.float32), fwd[:, 0])
+        cols = [c for c in range(27) if sc[:, c].max() > 0]
+        e_k = np.abs(k[:, cols] - ref64[:, cols]) / np.maximum(sc[:, cols], 1e-30)
+        e_r = np.abs(r32[:, cols] - ref64[:, cols]) / np.maximum(sc[:, cols], 1e-30)
+        assert e_k.max() <= 1.1 * e_r.max() + 1e-6
+        keep = [c for c in range(27) if not 7 <= c < 17]
+        assert np.array_equal(k[:, keep], ks[1][:, keep])
