@@ -16,6 +16,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -86,7 +87,42 @@ struct att_ctx {
     // Recorded after the last kernel that used d_pg or d_ord: a call on
     // another stream waits for it before reusing them.
     hipEvent_t scratchEv = nullptr;
+    // Device address of this GPU's `scratch` (att_score_adjoint_dev_noalloc).
+    void* noallocScratch = nullptr;
 };
+
+// Scratch of att_score_adjoint_dev_noalloc: the adjoint's per-workgroup
+// parameter-derivative rows (float) and their group sums (double), for at
+// most kNoAllocBlocks workgroups and 16 pair types (3 * 16^2 columns).  One
+// copy per GPU the library runs on, held until the process ends (13 MB).
+constexpr size_t kNoAllocBlocks = 4096;
+constexpr size_t kNoAllocCols = 3 * 16 * 16;
+constexpr size_t kNoAllocRowBytes = (sizeof(float) * kNoAllocBlocks * kNoAllocCols + 255) & ~(size_t)255;
+constexpr size_t kNoAllocBytes = kNoAllocRowBytes + sizeof(double) * att::kPgradGroups * kNoAllocCols;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// The device address of `scratch` on each GPU (the array's host-side name is
+// no device address): looked up once per device, with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void* g_scratchPtr[kMaxDevices] = {};
+
+hipError_t scratch_address(int device, void** ptr) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[device] = p;
+    }
+    *ptr = g_scratchPtr[device];
+    return hipSuccess;
+}
+
+}  // namespace
 
 namespace {
 
@@ -301,6 +337,11 @@ att_status att_ctx_create(int hip_device, att_ctx** out) {
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, hip_device) == hipSuccess && cus > 0)
         c->numCU = (unsigned)cus;
+    if (scratch_address(hip_device, &c->noallocScratch) != hipSuccess) {
+        (void)hipEventDestroy(c->scratchEv);
+        (void)hipStreamDestroy(c->stream);
+        return ATT_ERR_HIP;
+    }
     *out = c.release();
     return ATT_OK;
 }
@@ -703,6 +744,60 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
                                        reinterpret_cast<double*>(static_cast<char*>(c->d_pg) + rowBytes),
                                        d_pgrad, s));
     if (scratch) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
+    return ATT_OK;
+}
+
+att_status att_score_adjoint_dev_noalloc(att_ctx* c, att_handle hr, att_handle hl, att_handle hg,
+                                         att_handle hp, const float* d_dofs, int modes, size_t n,
+                                         const float* d_w, float* d_out, float* d_pgrad,
+                                         void* stream) {
+    if (!c) return ATT_ERR_INVALID;
+    if (!d_dofs || !d_out) return fail(c, ATT_ERR_INVALID, "null buffer");
+    if (n > 0x7FFFFFFFull) return fail(c, ATT_ERR_UNSUPPORTED, "too many poses in one call");
+    if (!c->noallocScratch) return fail(c, ATT_ERR_HIP, "no scratch address for this device");
+    ATT_HIP(c, hipSetDevice(c->device));
+    ProtObj *rec, *lig;
+    GridObj* g;
+    ParObj* p;
+    att_status st;
+    if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
+    if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
+    const bool pgrad = d_pgrad != nullptr;
+    if (pgrad && (!g->bin.oct || g->bin.nb != 2 * p->d.T))
+        return fail(c, ATT_ERR_INVALID, "parameter gradients need basis grids for this table (att_upload_basis)");
+    const unsigned cols = 3 * p->d.T * p->d.T;
+    if (pgrad && cols > kNoAllocCols)
+        return fail(c, ATT_ERR_UNSUPPORTED, "parameter gradients without allocation: at most 16 pair types");
+    unsigned slots = 0;
+    const int kWaves = pair_waves(lig->d.n);
+    const size_t dyn = adjoint_lds_bytes(rec->d, lig->d, p->d, pgrad, kWaves, &slots);
+    if (dyn + adjoint_static_lds(kWaves) > c->maxLdsPerBlock)
+        return fail(c, ATT_ERR_UNSUPPORTED, "pair table too large for the LDS of the adjoint kernel");
+    hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
+    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
+    if (pgrad && blocks > kNoAllocBlocks)
+        return fail(c, ATT_ERR_UNSUPPORTED, "more resident workgroups than the no-allocation scratch holds");
+    if (n == 0) {
+        if (pgrad) ATT_HIP(c, launch_fill_zero(d_pgrad, cols, s));
+        return ATT_OK;
+    }
+    PairDesc P{};
+    P.rec = rec->d; P.lig = lig->d;
+    P.inner = g->inner; P.outer = g->outer; P.nl = g->nl;
+    P.par = p->d;
+    P.bin = g->bin; P.bout = g->bout;
+    P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
+    P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
+    P.n = (unsigned)n; P.tabSlots = slots;
+    P.order = nullptr;  // input order: the spatial order needs sort scratch
+    // Every workgroup writes its whole row and the reduction every group sum,
+    // so the scratch needs no clearing between calls.
+    float* rows = static_cast<float*>(c->noallocScratch);
+    double* groups = reinterpret_cast<double*>(static_cast<char*>(c->noallocScratch) + kNoAllocRowBytes);
+    ATT_HIP(c, launch_adjoint(P, modes != 0, pgrad ? rows : nullptr, blocks, kWaves, dyn, s));
+    if (pgrad) ATT_HIP(c, launch_pgrad_reduce(rows, blocks, cols, groups, d_pgrad, s));
     return ATT_OK;
 }
 

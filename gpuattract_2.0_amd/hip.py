@@ -140,6 +140,7 @@ def lib():
         L.att_debug_rotmats.argtypes = [P, P, C.c_size_t, P]
         L.att_debug_sincosf.argtypes = [P, P, C.c_size_t, P, P]
         L.att_param_grad_rmin_emin.argtypes = [P, C.c_uint, C.c_int, P, P]
+        L.att_score_adjoint_dev_noalloc.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, P, P, P, P]
         L.att_sync.argtypes = [P, P]
         L.att_build_info.restype = C.c_char_p
         L.att_dev_alloc.argtypes = [P, C.c_size_t, C.POINTER(P)]
@@ -390,6 +391,11 @@ class Context:
         self._chk(self.L.att_score_adjoint_dev(self.h, hs["rec"], hs["lig"], hs["grid"],
                                                hs["params"], d_dofs, 1 if modes else 0, n, d_w,
                                                d_out, d_pgrad, stream))
+
+    def adjoint_dev_noalloc(self, hs, d_dofs, modes, n, d_w, d_out, d_pgrad=None, stream=None):
+        self._chk(self.L.att_score_adjoint_dev_noalloc(self.h, hs["rec"], hs["lig"], hs["grid"],
+                                                       hs["params"], d_dofs, 1 if modes else 0, n,
+                                                       d_w, d_out, d_pgrad, stream))
 
     def score_modes_dev(self, hs, d_dofs, n, d_out, stream=None):
         self._chk(self.L.att_score_modes_dev(self.h, hs["rec"], hs["lig"], hs["grid"],

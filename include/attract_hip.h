@@ -201,6 +201,22 @@ att_status att_score_adjoint_dev(att_ctx* ctx, att_handle rec, att_handle lig, a
                                  att_handle params, const float* d_dofs, int modes, size_t n,
                                  const float* d_w, float* d_out, float* d_pgrad, void* stream);
 
+/* att_score_adjoint_dev without any allocation: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops that forbid hipMalloc).  The parameter-gradient scratch is a
+ * fixed device array of the library (13 MB per GPU, kept until the process
+ * ends): at most 16 pair types and 4096 resident workgroups, else
+ * ATT_ERR_UNSUPPORTED; poses are taken in input order (no spatial order,
+ * which needs sort scratch), so the per-pose outputs equal
+ * att_score_adjoint_dev's and the parameter gradient equals it when that
+ * call runs in input order too (att_set_pose_order(ctx, 0)).  All calls on
+ * one GPU share the scratch: calls with parameter gradients that overlap on
+ * one GPU (other streams, other contexts) race. */
+att_status att_score_adjoint_dev_noalloc(att_ctx* ctx, att_handle rec, att_handle lig,
+                                         att_handle grid, att_handle params, const float* d_dofs,
+                                         int modes, size_t n, const float* d_w, float* d_out,
+                                         float* d_pgrad, void* stream);
+
 /* The parameter gradient with respect to the pair parameters the table is
  * derived from (fileIO/readFile.cpp:479-513): per table entry rmin =
  * sqrt(rmin2) and emin (the table's own field, minus the well depth), of
