@@ -81,7 +81,7 @@ def build_service():
         # relative run path: the tree is relocated on other machines
         link = ["-L", LIB, "-lattract_service", "-lattract_hip",
                 "-Wl,-rpath,$ORIGIN/../../../gpuattract_2.0_amd/lib"]
-        _run(cxx + [os.path.join(tdir, "server_semantics.cpp"), "-o",
+        _run(cxx + ["-I", d, os.path.join(tdir, "server_semantics.cpp"), "-o",
                     os.path.join(bin_dir, "server_semantics")])
         _run(cxx + ["-shared", os.path.join(tdir, "service_driver.cpp"), "-o",
                     os.path.join(bin_dir, "libservice_driver.so")] + link)

@@ -13,6 +13,8 @@
 #include <string>
 #include <utility>
 
+#include "InflightRing.h"
+
 namespace as {
 
 struct DeviceSet {
@@ -216,22 +218,30 @@ private:
     }
 
     // Marks finished items processed, in issue order; with block = true waits
-    // for the oldest one first.
+    // for the oldest one first.  A slot whose completion reports an error is
+    // freed with its item left unprocessed, then the error is raised.
     void retire(bool block) {
-        while (!_inflight.empty()) {
-            Slot& s = _slots[_inflight.front()];
-            // event queries and waits are thread-safe in the runtime: no lock,
-            // so a blocked worker does not hold up the others on this device
-            const att_status st = block ? att_event_sync(_dev.ctx, s.done) : att_event_query(_dev.ctx, s.done);
-            if (st != ATT_OK && st != ATT_NOT_READY) {
-                std::lock_guard<std::mutex> g(_dev.m);
-                check(_dev.ctx, st, "completion");
-            }
-            if (st == ATT_NOT_READY) return;
-            s.item->setProcessed();
-            s.item = nullptr;
-            _inflight.pop_front();
-            block = false;
+        att_status failed = ATT_OK;
+        const bool ok = retire_in_order(
+            _inflight, block,
+            [&](int k, bool b) {
+                // event queries and waits are thread-safe in the runtime: no
+                // lock, so a blocked worker does not hold up the others
+                const att_status st = b ? att_event_sync(_dev.ctx, _slots[k].done)
+                                        : att_event_query(_dev.ctx, _slots[k].done);
+                if (st == ATT_OK) return SlotStatus::Done;
+                if (st == ATT_NOT_READY) return SlotStatus::Pending;
+                failed = st;
+                return SlotStatus::Failed;
+            },
+            [&](int k) {
+                _slots[k].item->setProcessed();
+                _slots[k].item = nullptr;
+            },
+            [&](int k) { _slots[k].item = nullptr; });
+        if (!ok) {
+            std::lock_guard<std::mutex> g(_dev.m);
+            check(_dev.ctx, failed, "completion");
         }
     }
 

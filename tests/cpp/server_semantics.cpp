@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "as/Server.h"
+#include "InflightRing.h"
 
 using Types = GenericTypes<int, int, int>;
 
@@ -119,6 +120,47 @@ static bool run_sizes(const char* name, std::vector<unsigned> workers) {
     CHECK(ok, name);
     return ok;
 }
+
+// Keeps up to three items in flight like the MI355 pipeline; every item whose
+// first input is negative "fails" at completion (its slot is freed, the item
+// left unprocessed).  Completion is decided through retire_in_order.
+class FailingService : public MultiplyService {
+public:
+    itemProcessor_t createItemProcessor() override {
+        struct Ring {
+            std::deque<int> inflight;
+            workItem_t* slots[3] = {nullptr, nullptr, nullptr};
+        };
+        auto r = std::make_shared<Ring>();
+        return [r](workItem_t* item) {
+            auto retire = [&](bool block) {
+                as::retire_in_order(
+                    r->inflight, block,
+                    [&](int k, bool) {
+                        return r->slots[k]->inputBuffer()[0] < 0 ? as::SlotStatus::Failed : as::SlotStatus::Done;
+                    },
+                    [&](int k) {
+                        workItem_t* it = r->slots[k];
+                        for (unsigned i = 0; i < it->size(); ++i)
+                            it->resultBuffer()[i] = it->inputBuffer()[i] * *it->common();
+                        it->setProcessed();
+                        r->slots[k] = nullptr;
+                    },
+                    [&](int k) { r->slots[k] = nullptr; });
+            };
+            if (item) {
+                if (r->inflight.size() == 3) retire(true);
+                int k = 0;
+                while (r->slots[k]) ++k;
+                r->slots[k] = item;
+                r->inflight.push_back(k);
+            } else {
+                retire(true);
+            }
+            return !r->inflight.empty();
+        };
+    }
+};
 
 int main() {
     using server_t = as::Server<Types>;
@@ -226,6 +268,29 @@ int main() {
         }
         CHECK(ok, "several requests in flight, waited out of order");
     }
+    {
+        // A failed completion frees its slot: the failed request times out in
+        // wait(), the others complete, and the server shuts down.
+        server_t s(std::make_shared<FailingService>());
+        s.setItemSize(10);
+        s.setWaitTime(200);
+        s.createWorkers(2);
+        std::vector<int> good(100, 3), bad(100, 3), outg(100, 0), outb(100, 0);
+        bad[50] = -1;  // the item holding inputs 50..59 fails
+        request_t rg, rb;
+        rg.setDataAndSize(good.data(), good.size());
+        rg.setCommon(2);
+        rb.setDataAndSize(bad.data(), bad.size());
+        rb.setCommon(2);
+        s.submit(rg);
+        s.submit(rb);
+        s.wait(rg, outg.data());
+        CHECK(outg[0] == 6 && outg[99] == 6, "failed slots: other requests complete");
+        CHECK(error_of([&] { s.wait(rb, outb.data()); }) ==
+                  "Request takes too long to finish for unknown reasons",
+              "failed slots: the failed request times out");
+    }
+    CHECK(true, "failed slots: server shut down");
     std::printf("%s\n", failures ? "FAILED" : "ALL OK");
     return failures ? 1 : 0;
 }
