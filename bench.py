@@ -6,8 +6,8 @@ pair-parameter table (rc, ac, emin of every type pair, summed over poses),
 DOFs resident in HBM.  One step = k_adjoint over all poses + the fixed-order
 reduction of the per-workgroup parameter rows.  Reported beside it: the
 adjoint without parameter gradients, the fused forward kernel with the
-reference's force semantics, and the plain per-item path (host buffers, 100
-poses per call, as the drop-in service issues them).
+reference's force semantics, the plain per-item path (host buffers, 100 poses
+per call) and the drop-in as::Server path.
 
 Workload (default): the flexible-scoring configuration — a 3000-bead receptor
 and an 800-bead ligand (seeded synthetic bodies, 12 bead types, 20% charged),
@@ -15,12 +15,19 @@ and an 800-bead ligand (seeded synthetic bodies, 12 bead types, 20% charged),
 neighbour list with a 10 A plateau, 100,000 poses.  One step scores all poses
 with DOFs already resident in HBM.
 
+The same line carries, under "batched", the 64 uneven pairs scored by one
+graph replay (against one launch per pair and against the drop-in Server
+path), and under "serving", the docking-job stream's p50/p99 latency per
+arrival rate (--no-batched / --no-serve skip them).
+
 Prints one JSON line with the metric, a roofline object for the dominant
-kernel (algorithmic bytes per launch / launch time from HIP events on the
-launch stream) and the CPU baseline: the reference's own CPU path (compiled
-from its sources into oracle/_ref) on the host cores, worker threads taking
-items of 100 poses as the reference's server does, over a bounded sample; the
-oracle restatement stands in when the compiled reference is absent.
+kernel (algorithmic bytes per pose of the scoring path x poses / launch time
+from HIP events on the launch stream), the CPU baseline — the reference's own
+CPU path (compiled from its sources into oracle/_ref) on every host core,
+worker threads taking items of 100 poses as the reference's server does, over
+a bounded sample; the oracle restatement stands in when the compiled
+reference is absent — and "parity": the timed adjoint's results against the
+reference's energies and the oracle's gradients.
 """
 import argparse
 import importlib.util
@@ -32,6 +39,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "gpuattract_2.0_amd")
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+METRIC = "poses/sec (energy+DOF gradient) on 1 MI355X, batched+adjoint vs plain, at 1e-5 rel err"
 
 
 def _package():
@@ -46,8 +54,38 @@ def _package():
     return mod
 
 
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    return oracle_py
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (as lscpu prints it) and its logical cores."""
+    name = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                name = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 1
+
+
+def plain_rel(np, x, y):
+    """|x - y| / |y| elementwise, 0 where the bits agree (inf and nan included)."""
+    x = np.asarray(x, np.float64)
+    y = np.asarray(y, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.abs(x - y) / np.abs(y)
+    r[x == y] = 0.0
+    r[np.isnan(x) & np.isnan(y)] = 0.0
+    return r
 
 
 def make(args, W):
@@ -58,11 +96,10 @@ def make(args, W):
 
 
 def cpu_baseline(wl, args):
-    """The reference's CPU path (or, without it, the oracle restatement) on the
-    host cores over a bounded sample of the same poses."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    """The reference's CPU path (or, without it, the oracle restatement) on
+    every host core over a bounded sample of the same poses."""
+    O = _oracle()
+    model, threads = cpu_model()
     n = min(args.cpu_poses, wl.dofs.shape[0])
     d = wl.dofs[:n]
     if O.ref_available():
@@ -74,9 +111,43 @@ def cpu_baseline(wl, args):
         t0 = time.perf_counter()
         res = f(wl, d, nthreads=threads)
         dt = time.perf_counter() - t0
-    return dict(value=n / dt, unit="poses/s", cores=threads, kind=kind,
+    return dict(value=n / dt, unit="poses/s", cores=threads, kind=kind, cpu_model=model,
                 sample=f"first {n} poses of the same workload (energy + forces, the reference's "
                        f"gradient), {threads} worker threads, items of 100 poses, {dt:.1f} s"), res
+
+
+def adjoint_parity(np, O, wl, got, ref, k):
+    """The timed adjoint's output `got` (n, 7 | 27) against the reference's
+    forward energies `ref` (its first rows), against the oracle's FP32 adjoint
+    in the kernel's summation order (bit for bit) and against the FP64 adjoint
+    (errors scaled by the sum of the absolute terms of each component)."""
+    ow = got.shape[1]
+    m = ref.shape[0]
+    e_rel = plain_rel(np, got[:m, 0], ref[:, 0])
+    t0 = time.perf_counter()
+    ko = O.adjoint_kernel_order(wl, wl.dofs[:k], nthreads=os.cpu_count() or 1)
+    ko = ko[:, :ow].astype(np.float32)
+    bits = got[:k].view(np.uint32) == ko.view(np.uint32)
+    r64, sc = O.adjoint(wl, wl.dofs[:k], scale=True)
+    cols = [c for c in range(1, ow) if sc[:, c].max() > 0]
+    g = got[:k, cols].astype(np.float64)
+    scaled = np.abs(g - r64[:, cols]) / np.maximum(sc[:, cols], 1e-30)
+    e64 = plain_rel(np, got[:k, 0], r64[:, 0])
+    out = dict(energy_vs_reference_poses=int(m),
+               energy_vs_reference_max_rel_err=float(e_rel.max()),
+               energy_vs_reference_bit_exact_share=float(np.mean(got[:m, 0].view(np.uint32) ==
+                                                                 ref[:, 0].astype(np.float32).view(np.uint32))),
+               kernel_order_oracle_poses=int(k),
+               kernel_order_oracle_bit_exact=bool(bits.all()),
+               kernel_order_oracle_bit_exact_share=float(bits.mean()),
+               energy_vs_fp64_max_rel_err=float(e64.max()),
+               gradient_vs_fp64_max_scaled_err=float(scaled.max()),
+               gradient_vs_fp64_median_scaled_err=float(np.median(scaled)))
+    log(f"parity: energy vs reference ({m} poses) max rel {out['energy_vs_reference_max_rel_err']:.2e}; "
+        f"kernel-order oracle ({k} poses) bit-exact {out['kernel_order_oracle_bit_exact']}; gradient "
+        f"vs FP64 max scaled {out['gradient_vs_fp64_max_scaled_err']:.2e} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    return out
 
 
 def main():
@@ -90,11 +161,16 @@ def main():
     ap.add_argument("--min-check", type=int, default=1000,
                     help="minimize: poses compared with the oracle's VA13 on the CPU")
     ap.add_argument("--pairs", type=int, default=64, help="batched workload: number of pairs")
+    ap.add_argument("--batched-steps", type=int, default=10, help="graph replays timed (batched)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-jobs", type=int, default=40, help="serve: jobs per arrival rate")
     ap.add_argument("--serve-rates", default="5,20,50,200", help="serve: arrival rates, jobs/s")
     ap.add_argument("--serve-evals", type=int, default=200, help="serve: VA13 evaluation limit")
     ap.add_argument("--poses", type=int, default=100000)
     ap.add_argument("--cpu-poses", type=int, default=50000)
+    ap.add_argument("--check-poses", type=int, default=512,
+                    help="poses of the timed adjoint compared with the oracle's adjoints")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-server", action="store_true",
                     help="skip timing the drop-in Server path")
@@ -106,7 +182,8 @@ def main():
                          "(calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
+        print(json.dumps({"error": "this build measures one GPU only (the path runs on one GPU's "
+                                   "share of the poses); run with --gpus 1"}))
         sys.exit(2)
 
     import numpy as np
@@ -185,16 +262,29 @@ def main():
     avg_launch = float(np.mean(launch_ms)) / 1e3
     value = n * args.steps / wall
     log(f"timed {args.steps} steps: {wall:.3f} s, {value:.4g} poses/s, launch avg {avg_launch*1e3:.3f} ms")
+    timed = ctx.to_host(d_grad, (n, out_w))
 
     copy_calibration(args, ctx, sh)
-    bpp, bpp_sd = traffic.bytes_per_pose(wl, adjoint=True, pgrad=pgrad)
+    # roofline numerator: the scoring path's algorithmic bytes per pose (bead
+    # attributes, grid corners, neighbour-list descriptors and entries, mode
+    # vectors, DOFs in and results out); the adjoint's own reads (it skips the
+    # receptor-on-ligand-grid lookups) and the parameter gradient's basis
+    # octets are reported beside it
+    bpp, bpp_sd = traffic.bytes_per_pose(wl)
+    adj_bpp, _ = traffic.bytes_per_pose(wl, adjoint=True)
+    basis_bpp = (traffic.bytes_per_pose(wl, adjoint=True, pgrad=True)[0] - adj_bpp) if pgrad else 0.0
     achieved = bpp * n / avg_launch
+    prof = profiled_traffic(args.workload) if pgrad else None
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=profiled_traffic(args.workload) if pgrad else None,
-                    bytes_per_pose=bpp, kernel="k_adjoint")
+                    frac=achieved / HBM_PEAK, traffic=prof["bytes"] if prof else None,
+                    traffic_source=prof["source"] if prof else None,
+                    kernel=f"k_adjoint<{5 if wl.modes else 0}> (+ parameter-gradient row reduction)",
+                    bytes_per_pose=bpp, bytes_per_pose_sd=bpp_sd, poses_per_launch=n,
+                    avg_launch_ms=avg_launch * 1e3,
+                    adjoint_read_bytes_per_pose=adj_bpp,
+                    basis_octet_bytes_per_pose=basis_bpp,
+                    frac_with_adjoint_and_basis_reads=(adj_bpp + basis_bpp) * n / avg_launch / HBM_PEAK)
 
-    # The plain per-item path: host buffers, 100 poses per call (the reference's
-    # item size, server/Server.h:39), as the drop-in service issues them.
     # the same adjoint from host buffers in one call (DOFs over PCIe in, results
     # and parameter gradients out): the PCIe-inclusive rate, never `value`
     ctx.adjoint(hs, wl.dofs[:1000], pgrad=pgrad, T=T)
@@ -203,6 +293,8 @@ def main():
     host_adj = n / (time.perf_counter() - t0)
     log(f"adjoint from host buffers, one call: {host_adj:.4g} poses/s")
 
+    # The plain per-item path: host buffers, 100 poses per call (the reference's
+    # item size, server/Server.h:39), as the drop-in service issues them.
     m = min(args.plain_poses, n)
     host_score = ctx.score_modes if wl.modes else ctx.score_rigid
     host_score(hs, wl.dofs[:100])
@@ -223,17 +315,29 @@ def main():
         log("drop-in Server path, items of 100, 8 requests in flight: " + ", ".join(
             f"{w} worker(s) {r:.4g} poses/s" for w, r in server_by_workers.items()))
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu:
         cpu, ref = cpu_baseline(wl, args)
-        gpu = ctx.to_host(d_out, (n, out_w))[:ref.shape[0]]
-        fin = np.isfinite(ref[:, 0]) & np.isfinite(gpu[:, 0])
-        rel = np.abs(gpu[fin, 0] - ref[fin, 0]) / np.maximum(np.abs(ref[fin, 0]), 1e-30)
-        cpu["forward_vs_reference_energy_median_rel_err"] = float(np.median(rel))
-        log(f"cpu baseline: {cpu['value']:.4g} poses/s on {cpu['cores']} threads")
+        fwd = ctx.to_host(d_out, (n, out_w))[:ref.shape[0]]
+        cpu["forward_vs_reference_energy_max_rel_err"] = float(plain_rel(np, fwd[:, 0], ref[:, 0]).max())
+        log(f"cpu baseline: {cpu['value']:.4g} poses/s on {cpu['cores']} threads ({cpu['cpu_model']})")
+        parity = adjoint_parity(np, _oracle(), wl, timed, ref, min(args.check_poses, n))
+    ctx.close()
+
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args, np, hip, traffic, W)
+    serving = None
+    if not args.no_serve:
+        rows, st = serve_rows(args, np, W)
+        serving = dict(stream=f"{st['jobs']} docking jobs, {st['receptors']} receptors, {st['poses']} "
+                              f"poses, VA13 <= {args.serve_evals} evaluations",
+                       rates=[{k: r[k] for k in ("rate_jobs_per_s", "p50_ms", "p99_ms", "poses_per_s",
+                                                 "pose_evaluations_per_s", "jobs_per_s", "max_running")}
+                              for r in rows])
 
     line = {
-        "metric": "poses/sec (energy+DOF gradient) on 1 MI355X, batched+adjoint vs plain, at 1e-5 rel err",
+        "metric": METRIC,
         "value": value,
         "unit": "poses/s",
         "n_gpus": 1,
@@ -264,9 +368,11 @@ def main():
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "parity": parity,
+        "batched": batched,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
-    ctx.close()
 
 
 def run_minimize(args, np, hip, W):
@@ -307,7 +413,7 @@ def run_minimize(args, np, hip, W):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
         from concurrent.futures import ThreadPoolExecutor
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads = os.cpu_count() or 1
         m = min(n, args.min_check)
         chunks = [(a, min(m, a + 8)) for a in range(0, m, 8)]
         t1 = time.perf_counter()
@@ -343,7 +449,7 @@ def run_minimize(args, np, hip, W):
     ctx.close()
 
 
-def run_serve(args, np, W):
+def serve_rows(args, np, W):
     """Serving stream: docking jobs (receptor 500-8000 beads, ligand 100-2000,
     0-10 modes per body, 500-20000 poses, VA13 capped at --serve-evals
     evaluations; 30% of jobs reuse an earlier job's receptor) arrive as a
@@ -407,6 +513,12 @@ def run_serve(args, np, W):
             f"{row['pose_evaluations_per_s']:.4g} evaluations/s, p50 {row['p50_ms']:.0f} ms, "
             f"p99 {row['p99_ms']:.0f} ms, up to {row['max_running']} jobs at once, "
             f"{row['grid_hits']} grid hits / {row['grid_misses']} builds")
+    return per_rate, dict(jobs=len(jobs), receptors=len(recs), poses=total_poses)
+
+
+def run_serve(args, np, W):
+    """The serving stream on its own (serve_rows), the best rate as `value`."""
+    per_rate, st = serve_rows(args, np, W)
     best = max(per_rate, key=lambda r: r["poses_per_s"])
     line = {
         "metric": "docking jobs served: sustained minimized poses/s with p50/p99 job latency",
@@ -414,8 +526,8 @@ def run_serve(args, np, W):
         "warmup": 0, "ms_per_step": best["wall_s"] * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32 scoring, fp64 minimizer",
         "data": "synthetic (seeded stream of bodies, poses, modes; grids built on the device)",
-        "config": {"workload": f"serving stream, {len(jobs)} jobs, {len(recs)} receptors, "
-                               f"{total_poses} poses, VA13 <= {args.serve_evals} evaluations",
+        "config": {"workload": f"serving stream, {st['jobs']} jobs, {st['receptors']} receptors, "
+                               f"{st['poses']} poses, VA13 <= {args.serve_evals} evaluations",
                    "rates": per_rate},
     }
     print(json.dumps(line), flush=True)
@@ -469,33 +581,6 @@ def server_pass_seconds(np, hip, wl, dofs, workers, requests=8, item=100, reps=3
     return sec.value
 
 
-def server_rate_batched(np, hip, W, ctx, staged, total):
-    """The same pairs through the drop-in path, one Server per pair (2 workers,
-    items of 100 poses, 8 requests in flight), pair after pair; the ligand
-    grids the modes service needs are generated on the device here (the
-    batched adjoint does not use them).  Poses/s over the timed passes."""
-    t0 = time.perf_counter()
-    secs = 0.0
-    for k, s in enumerate(staged):
-        wl = s["wl"]
-        if wl.modes and wl.lig_inner is None:
-            gli, glo = W.pair_lig_grid_geoms(wl)
-            T = wl.params.T
-            wl.lig_inner = W.IGrid(gli, T + 1, ctx.generate_grid(wl.lig, wl.params, gli, src_is_ligand=True))
-            wl.lig_outer = W.IGrid(glo, T + 1, ctx.generate_grid(wl.lig, wl.params, glo, src_is_ligand=True))
-        sec = server_pass_seconds(np, hip, wl, wl.dofs, 2, reps=1)
-        if wl.modes:
-            wl.lig_inner = wl.lig_outer = None  # host memory: one pair's grids at a time
-        if sec is None:
-            return None
-        secs += sec
-        if k % 16 == 15:
-            log(f"  drop-in Server path: {k + 1} pairs ({time.perf_counter() - t0:.0f} s)")
-    rate = total / secs
-    log(f"drop-in Server path (2 workers, items of 100), pair after pair: {rate:.4g} poses/s")
-    return rate
-
-
 def copy_calibration(args, ctx, sh):
     """The plain copy kernel over --copy-calibration bytes (read once, written
     once), so that a counter run can scale FETCH_SIZE / WRITE_SIZE."""
@@ -512,57 +597,70 @@ def copy_calibration(args, ctx, sh):
 
 def profiled_traffic(workload):
     """HBM bytes per launch (per graph replay for the batched workload) from
-    the calibrated FETCH_SIZE / WRITE_SIZE counter runs kept in profiles/."""
-    prof = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    the calibrated FETCH_SIZE / WRITE_SIZE counter runs kept in profiles/:
+    {bytes, source} or None."""
+    name = f"traffic_{workload}.json"
     try:
-        return json.load(open(prof)).get("hbm_bytes_per_launch")
+        rec = json.load(open(os.path.join(ROOT, "profiles", name)))
     except (OSError, ValueError):
         return None
+    return dict(bytes=rec.get("hbm_bytes_per_launch"),
+                source=f"profiles/{name}: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of "
+                       f"{rec.get('kernel', 'the kernel')} on {rec.get('workload', 'this workload')}, "
+                       f"scaled by the counts of a copy kernel of known size "
+                       f"(factors {rec.get('fetch_factor')} / {rec.get('write_factor')})")
 
 
-def run_batched(args, np, hip, traffic, W):
-    """Uneven pairs (receptor 500-5000 beads, ligand 100-1500, 0-10 modes per
-    body, 1000-50000 poses, all log-uniform except modes) scored by the batched
-    adjoint: buckets tuned by the autotuner, launches captured in a hipGraph;
-    one step replays the graph over every pose of every pair."""
+def batched_leg(args, np, hip, traffic, W, check=0):
+    """The 64 uneven pairs (receptor 500-5000 beads, ligand 100-1500, 0-10
+    modes per body, 1000-50000 poses, all log-uniform except modes) scored by
+    the batched adjoint: buckets tuned by the autotuner, launches captured in
+    a hipGraph; one step replays the graph over every pose of every pair.
+    Receptor grids are built on the device.  Beside it: one device launch per
+    pair, the plain item-100 path and the drop-in Server path pair after pair
+    (each pair's grids fetched to the host for it, one pair at a time).  With
+    check > 0, the first `check` poses of every pair are compared with the
+    oracle's FP32 adjoint in the kernel's summation order (bit for bit)."""
     t0 = time.perf_counter()
-    pairs = W.make_batch(20261204, args.pairs,
-                         progress=lambda k, wl: k % 8 == 0 and log(f"  {k} pairs built ({wl.name})"))
+    pairs = W.make_batch(20261204, args.pairs, host_grids=False)
     total = sum(p.dofs.shape[0] for p in pairs)
-    log(f"{len(pairs)} pairs built in {time.perf_counter() - t0:.1f} s, {total} poses")
+    log(f"batched: {len(pairs)} pairs made in {time.perf_counter() - t0:.1f} s, {total} poses")
     ctx = hip.Context(0)
+    t0 = time.perf_counter()
     staged = []
     for wl in pairs:
         hs = ctx.upload_workload(wl)
         n, ow = wl.dofs.shape[0], (27 if wl.modes else 7)
         staged.append(dict(wl=wl, hs=hs, d_dofs=ctx.to_device(wl.dofs), n=n, ow=ow,
                            d_out=ctx.dev_alloc(n * ow * 4)))
-    log("uploaded")
+    log(f"batched: grids built on the device and poses uploaded in {time.perf_counter() - t0:.1f} s")
     b = ctx.batch([dict(hs=s["hs"], d_dofs=s["d_dofs"].ptr, d_out=s["d_out"].ptr, n=s["n"],
                         modes=s["wl"].modes) for s in staged])
     sh = ctx.stream()
     t0 = time.perf_counter()
     b.tune(sh)
     buckets = b.buckets()
-    log(f"tuned in {time.perf_counter() - t0:.1f} s: " + ", ".join(
+    log(f"batched: tuned in {time.perf_counter() - t0:.1f} s: " + ", ".join(
         f"M{k['modeCapacity']}/size{k['sizeClass']}: {k['numPairs']} pairs, {k['wavesPerPose']} "
         f"waves/pose, {k['posesPerTile']} poses/tile, {k['ms']:.2f} ms" for k in buckets))
     for _ in range(max(1, args.warmup)):
         b.run(sh)
     ctx.sync(sh)
-    starts = [ctx.event() for _ in range(args.steps)]
-    ends = [ctx.event() for _ in range(args.steps)]
+    steps = args.batched_steps
+    starts = [ctx.event() for _ in range(steps)]
+    ends = [ctx.event() for _ in range(steps)]
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         ctx.record(starts[k], sh)
         b.run(sh)
         ctx.record(ends[k], sh)
     ctx.sync(sh)
     wall = time.perf_counter() - t0
     launch = float(np.mean([ctx.elapsed_ms(a, c) for a, c in zip(starts, ends)])) / 1e3
-    value = total * args.steps / wall
-    log(f"timed {args.steps} graph replays: {value:.4g} poses/s, {launch * 1e3:.3f} ms per replay")
+    value = total * steps / wall
+    log(f"batched: timed {steps} graph replays: {value:.4g} poses/s, {launch * 1e3:.3f} ms per replay")
     copy_calibration(args, ctx, sh)
+    outs = [ctx.to_host(s["d_out"], (s["n"], s["ow"]))[:max(check, 1)] for s in staged]
 
     # the same work pair by pair: one device adjoint launch per pair
     e0, e1 = ctx.event(), ctx.event()
@@ -581,46 +679,111 @@ def run_batched(args, np, hip, traffic, W):
             ctx.adjoint(s["hs"], d[a:a + 100])
         done += d.shape[0]
     plain = done / (time.perf_counter() - t0)
-    log(f"per-pair device launches: {per_pair:.4g} poses/s; plain item-100 path: {plain:.4g} poses/s")
-    server = None if args.no_server else server_rate_batched(np, hip, W, ctx, staged, total)
+    log(f"batched: per-pair device launches {per_pair:.4g} poses/s; plain item-100 path {plain:.4g} poses/s")
 
-    bpp = [traffic.bytes_per_pose(s["wl"], sample=64, adjoint=True)[0] for s in staged]
-    algo = sum(bp * s["n"] for bp, s in zip(bpp, staged))
-    roofline = dict(bound="hbm", achieved=algo / launch / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=algo / launch / HBM_PEAK, traffic=profiled_traffic("batched"),
-                    bytes_per_pose=algo / total, kernel="k_adjoint_batch")
-    cpu = None
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py as O
-        from concurrent.futures import ThreadPoolExecutor
-        threads = max(1, min(16, os.cpu_count() or 1))
-        k = 160
-        t0 = time.perf_counter()
-        with ThreadPoolExecutor(threads) as ex:
-            list(ex.map(lambda s: O.adjoint(s["wl"], s["wl"].dofs[:k], f32=True), staged))
-        dt = time.perf_counter() - t0
-        cpu = dict(value=k * len(staged) / dt, unit="poses/s", cores=threads, kind="port",
-                   sample=f"first {k} poses of every pair, the oracle's FP32 adjoint (energy + exact "
-                          f"gradient), pairs on {threads} threads, {dt:.1f} s")
-    line = {
-        "metric": "poses/sec (energy+DOF gradient) on 1 MI355X, batched+adjoint vs plain, at 1e-5 rel err",
-        "value": value, "unit": "poses/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32",
-        "data": "synthetic (seeded generator: bodies, pair tables, grids, poses, modes)",
-        "config": {"workload": f"batched {len(staged)} uneven pairs", "pairs": len(staged),
-                   "poses": total, "path": "batched adjoint, autotuned buckets, hipGraph replay",
-                   "buckets": buckets, "per_pair_launches_poses_per_s": per_pair,
-                   "plain_item100_poses_per_s": plain, "batched_vs_plain": value / plain,
-                   "server_item100_poses_per_s": server,
-                   "batched_vs_server": value / server if server else None,
-                   "batched_vs_per_pair": value / per_pair},
-        "roofline": roofline, "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
+    # pair after pair, with the pair's grids on the host: algorithmic bytes,
+    # the drop-in Server path and the oracle check
+    O = _oracle() if check else None
+    algo = algo_fwd = secs = 0.0
+    worst, exact, checked = 0.0, True, 0
+    t0 = time.perf_counter()
+    for k, s in enumerate(staged):
+        wl = s["wl"]
+        W.fill_grids_from_device(ctx, wl)
+        T = wl.params.T
+        gli, glo = W.pair_lig_grid_geoms(wl)
+        if wl.modes:
+            # the forward path's ligand grids (the adjoint does not read them)
+            wl.lig_inner = W.IGrid(gli, T + 1, ctx.generate_grid(wl.lig, wl.params, gli, src_is_ligand=True))
+            wl.lig_outer = W.IGrid(glo, T + 1, ctx.generate_grid(wl.lig, wl.params, glo, src_is_ligand=True))
+        algo += traffic.bytes_per_pose(wl, sample=64, adjoint=True)[0] * s["n"]
+        if wl.modes:
+            algo_fwd += traffic.bytes_per_pose(wl, sample=64)[0] * s["n"]
+        else:
+            algo_fwd += traffic.bytes_per_pose(wl, sample=64, adjoint=True)[0] * s["n"]
+        if not args.no_server and secs is not None:
+            sec = server_pass_seconds(np, hip, wl, wl.dofs, 2, reps=1)
+            secs = None if sec is None else secs + sec
+        if check:
+            waves = buckets[b.pair_bucket(k)]["wavesPerPose"]
+            c = min(check, s["n"])
+            ko = O.adjoint_kernel_order(wl, wl.dofs[:c], waves=waves,
+                                        nthreads=os.cpu_count() or 1)[:, :s["ow"]].astype(np.float32)
+            g = outs[k][:c]
+            exact &= bool(np.array_equal(g.view(np.uint32), ko.view(np.uint32)))
+            worst = max(worst, float(plain_rel(np, g[:, 0], ko[:, 0]).max()))
+            checked += c
+        # host memory: one pair's grids at a time
+        wl.rec_inner = W.IGrid(wl.rec_inner.geom, T + 1, None)
+        wl.rec_outer = W.IGrid(wl.rec_outer.geom, T + 1, None)
+        wl.rec_nl = W.NLGrid(wl.rec_nl.geom, wl.rec_nl.plateau, None, None)
+        wl.lig_inner = wl.lig_outer = None
+        if k % 16 == 15:
+            log(f"batched: {k + 1} pairs through the host-side passes ({time.perf_counter() - t0:.0f} s)")
+    server = total / secs if secs else None
+    if server:
+        log(f"batched: drop-in Server path (2 workers, items of 100), pair after pair: {server:.4g} poses/s")
+    roofline = dict(bound="hbm", achieved=algo_fwd / launch / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=algo_fwd / launch / HBM_PEAK, kernel="k_adjoint_batch",
+                    bytes_per_pose=algo_fwd / total, adjoint_read_bytes_per_pose=algo / total,
+                    frac_with_adjoint_reads=algo / launch / HBM_PEAK)
+    prof = profiled_traffic("batched")
+    if prof:
+        roofline.update(traffic=prof["bytes"], traffic_source=prof["source"])
+    out = dict(workload=f"batched {len(staged)} uneven pairs", pairs=len(staged), poses=total,
+               poses_per_s=value, ms_per_replay=launch * 1e3, replays=steps,
+               path="batched adjoint (energy + exact DOF and mode gradients), autotuned buckets, "
+                    "hipGraph replay",
+               buckets=buckets, per_pair_launches_poses_per_s=per_pair,
+               plain_item100_poses_per_s=plain, batched_vs_plain=value / plain,
+               server_item100_poses_per_s=server,
+               batched_vs_server=value / server if server else None,
+               batched_vs_per_pair=value / per_pair, roofline=roofline)
+    if check:
+        out["parity"] = dict(poses=checked, kernel_order_oracle_bit_exact=exact,
+                             energy_max_rel_err=worst)
+        log(f"batched: {checked} poses against the kernel-order oracle: bit-exact {exact}")
     b.close()
     ctx.close()
+    return out
+
+
+def run_batched(args, np, hip, traffic, W):
+    """The batched leg on its own line, the graph replay rate as `value`, with
+    the oracle's FP32 adjoint timed on the host as the CPU baseline."""
+    args.batched_steps = args.steps
+    res = batched_leg(args, np, hip, traffic, W, check=0 if args.no_cpu else 160)
+    cpu = None
+    if not args.no_cpu:
+        O = _oracle()
+        from concurrent.futures import ThreadPoolExecutor
+        model, threads = cpu_model()
+        pairs = W.make_batch(20261204, args.pairs, host_grids=False)
+        ctx = hip.Context(0)
+        k = 160
+        dt = 0.0
+        for wl in pairs:  # grids from the device, one pair at a time
+            W.fill_grids_from_device(ctx, wl)
+            d = wl.dofs[:k]
+            t0 = time.perf_counter()
+            with ThreadPoolExecutor(threads) as ex:
+                list(ex.map(lambda c: O.adjoint(wl, d[c:c + 10], f32=True), range(0, d.shape[0], 10)))
+            dt += time.perf_counter() - t0
+            wl.rec_inner = wl.rec_outer = wl.rec_nl = None
+        ctx.close()
+        cpu = dict(value=k * len(pairs) / dt, unit="poses/s", cores=threads, kind="port",
+                   cpu_model=model,
+                   sample=f"first {k} poses of every pair, the oracle's FP32 adjoint (energy + exact "
+                          f"gradient), items of 10 poses on {threads} threads, {dt:.1f} s")
+    line = {
+        "metric": METRIC, "value": res["poses_per_s"], "unit": "poses/s", "n_gpus": 1,
+        "steps": res["replays"], "warmup": args.warmup, "ms_per_step": res["ms_per_replay"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded generator: bodies, pair tables, grids, poses, modes)",
+        "config": {k: v for k, v in res.items() if k not in ("roofline", "parity")},
+        "roofline": res["roofline"], "cpu_baseline": cpu, "parity": res.get("parity"),
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

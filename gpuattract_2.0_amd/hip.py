@@ -342,9 +342,16 @@ class Context:
         return out.value
 
     def upload_workload(self, wl):
-        """Upload everything a workload needs; returns a dict of handles."""
-        hs = dict(rec=self.upload_protein(wl.rec), lig=self.upload_protein(wl.lig),
-                  grid=self.upload_grid(wl.rec_inner, wl.rec_outer, wl.rec_nl),
+        """Upload everything a workload needs; returns a dict of handles.  A
+        workload without host grids (make_workload(host_grids=False)) gets its
+        receptor grids built on the device."""
+        if wl.rec_inner.data is None:
+            m = wl.meta
+            grid = self.build_grid(wl.rec, wl.params, wl.rec_inner.geom, wl.rec_outer.geom,
+                                   m["plateau"], m["neighbourdis"])
+        else:
+            grid = self.upload_grid(wl.rec_inner, wl.rec_outer, wl.rec_nl)
+        hs = dict(rec=self.upload_protein(wl.rec), lig=self.upload_protein(wl.lig), grid=grid,
                   params=self.upload_params(wl.params))
         if wl.lig_inner is not None:
             hs["lig_grid"] = self.upload_grid(wl.lig_inner, wl.lig_outer, None)

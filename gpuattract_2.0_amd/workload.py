@@ -314,10 +314,14 @@ def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_ty
                   spacing=0.875, margin=10.0, outer_spacing=2.6, plateau=10.0,
                   neighbourdis=10.8, charged_frac=0.2, type0_frac=0.0, shell=None,
                   dielec=1, shape=8, with_lig_grid=None, repulsive_frac=0.2,
-                  with_basis=False):
+                  with_basis=False, host_grids=True):
     """A receptor-ligand pair with grids and poses, as described by the
     benchmark configurations (receptor density of 1000 beads per 25 A sphere,
-    ligand 300 per 15 A sphere, 3 A minimum spacing, 12 types, 20% charged)."""
+    ligand 300 per 15 A sphere, 3 A minimum spacing, 12 types, 20% charged).
+    host_grids=False leaves the receptor's grids and neighbour list unbuilt
+    (geometry only, data None): Context.upload_workload then builds them on
+    the device, bit-identical to the host generator (fill_grids_from_device
+    fetches them for host-side checks)."""
     R_rec = scaled_radius(n_rec, DENSITY_REF)
     R_lig = scaled_radius(n_lig, LIG_REF)
     rec = make_body(seed, 0, n_rec, R_rec, num_types, charged_frac, type0_frac)
@@ -330,10 +334,14 @@ def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_ty
         shell = (0.75 * (R_rec + R_lig), 1.0 * (R_rec + R_lig))
     r_lo, r_hi = shell
     gi = inner_geom(rec, spacing, margin)
-    rec_inner = build_grid(gi, rec, params, False, plateau)
     go = cube_geom((0.0, 0.0, 0.0), r_hi + R_lig + 2.0, outer_spacing)
-    rec_outer = build_grid(go, rec, params, False, plateau)
-    rec_nl = build_nl(gi, rec, neighbourdis, plateau)
+    if host_grids:
+        rec_inner = build_grid(gi, rec, params, False, plateau)
+        rec_outer = build_grid(go, rec, params, False, plateau)
+        rec_nl = build_nl(gi, rec, neighbourdis, plateau)
+    else:
+        rec_inner, rec_outer = IGrid(gi, num_types + 1, None), IGrid(go, num_types + 1, None)
+        rec_nl = NLGrid(gi, plateau, None, None)
     modes = (M_rec > 0 or M_lig > 0) if with_lig_grid is None else with_lig_grid
     dofs6 = make_poses(seed, n_poses, r_lo, r_hi)
     wl = Workload(name, rec, lig, params, rec_inner, rec_outer, rec_nl, dofs6)
@@ -354,6 +362,22 @@ def make_workload(name, seed, n_rec, n_lig, n_poses, M_rec=0, M_lig=0, *, num_ty
     wl.meta = dict(seed=seed, n_rec=n_rec, n_lig=n_lig, n_poses=n_poses, M_rec=M_rec,
                    M_lig=M_lig, R_rec=R_rec, R_lig=R_lig, shell=shell, spacing=spacing,
                    outer_spacing=outer_spacing, plateau=plateau, neighbourdis=neighbourdis)
+    return wl
+
+
+def fill_grids_from_device(ctx, wl):
+    """The receptor grids and neighbour list of a workload made with
+    host_grids=False, generated on the device (att_generate_grid /
+    att_generate_nl, the host generator's values bit for bit) into host
+    arrays, for the CPU oracle."""
+    m = wl.meta
+    if wl.rec_inner.data is None:
+        wl.rec_inner = IGrid(wl.rec_inner.geom, wl.params.T + 1,
+                             ctx.generate_grid(wl.rec, wl.params, wl.rec_inner.geom, m["plateau"]))
+        wl.rec_outer = IGrid(wl.rec_outer.geom, wl.params.T + 1,
+                             ctx.generate_grid(wl.rec, wl.params, wl.rec_outer.geom, m["plateau"]))
+        desc, lst = ctx.generate_nl(wl.rec, wl.rec_nl.geom, m["neighbourdis"])
+        wl.rec_nl = NLGrid(wl.rec_nl.geom, m["plateau"], desc, lst)
     return wl
 
 

@@ -90,3 +90,21 @@ def test_grid_generation_errors(ctx, W):
         ctx.generate_grid(rec, p, g, plateau=0.0)
     with pytest.raises(hip.AttError):
         ctx.generate_nl(rec, g, -2.0)
+
+
+def test_workload_without_host_grids(ctx, W):
+    """make_workload(host_grids=False): upload_workload builds the receptor
+    grids on the device and scores exactly like the host-built workload;
+    fill_grids_from_device returns the host generator's arrays."""
+    args = ("no host grids", 37, 800, 250, 200, 2, 3)
+    wh = W.make_workload(*args, with_lig_grid=True)
+    wd = W.make_workload(*args, with_lig_grid=True, host_grids=False)
+    assert wd.rec_inner.data is None and wd.rec_nl.desc is None
+    hh, hd = ctx.upload_workload(wh), ctx.upload_workload(wd)
+    assert np.array_equal(ctx.score_modes(hd, wd.dofs), ctx.score_modes(hh, wh.dofs))
+    assert np.array_equal(ctx.adjoint(hd, wd.dofs), ctx.adjoint(hh, wh.dofs))
+    W.fill_grids_from_device(ctx, wd)
+    for a, b in ((wd.rec_inner, wh.rec_inner), (wd.rec_outer, wh.rec_outer)):
+        assert np.array_equal(a.data.view(np.uint32), b.data.view(np.uint32))
+    assert np.array_equal(wd.rec_nl.desc, wh.rec_nl.desc)
+    assert np.array_equal(wd.rec_nl.list, wh.rec_nl.list)
