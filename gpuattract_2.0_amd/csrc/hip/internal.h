@@ -16,7 +16,7 @@ hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, 
 
 size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots);
 
-// `waves` waves per pose: 1 or 2.
+// `waves` waves per pose: 1, 2, 4 or 8.
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,

@@ -534,9 +534,17 @@ hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
 #define ATT_LAUNCH_MODES(MMV, WV)                                                                 \
     hipLaunchKernelGGL((k_modes<MMV, WV>), dim3(n), dim3(64 * WV), lds, s, rec, lig, rInner, rOuter, \
                        nl, lInner, lOuter, par, dofs, res, n, order)
-    if (mm == 0) { if (waves > 1) ATT_LAUNCH_MODES(0, 2); else ATT_LAUNCH_MODES(0, 1); }
-    else if (mm <= 5) { if (waves > 1) ATT_LAUNCH_MODES(5, 2); else ATT_LAUNCH_MODES(5, 1); }
-    else { if (waves > 1) ATT_LAUNCH_MODES(ATT_MODES_MAX, 2); else ATT_LAUNCH_MODES(ATT_MODES_MAX, 1); }
+#define ATT_LAUNCH_MODES_W(MMV)                                                                  \
+    switch (waves) {                                                                              \
+    case 8: ATT_LAUNCH_MODES(MMV, 8); break;                                                      \
+    case 4: ATT_LAUNCH_MODES(MMV, 4); break;                                                      \
+    case 2: ATT_LAUNCH_MODES(MMV, 2); break;                                                      \
+    default: ATT_LAUNCH_MODES(MMV, 1); break;                                                     \
+    }
+    if (mm == 0) ATT_LAUNCH_MODES_W(0)
+    else if (mm <= 5) ATT_LAUNCH_MODES_W(5)
+    else ATT_LAUNCH_MODES_W(ATT_MODES_MAX)
+#undef ATT_LAUNCH_MODES_W
 #undef ATT_LAUNCH_MODES
     return hipGetLastError();
 }

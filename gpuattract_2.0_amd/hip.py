@@ -121,6 +121,7 @@ def lib():
         L = C.CDLL(LIBPATH)
         L.att_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.att_set_pose_order.argtypes = [P, C.c_int]
+        L.att_set_modes_waves.argtypes = [P, C.c_int]
         L.att_ctx_destroy.argtypes = [P]
         L.att_ctx_destroy.restype = None
         L.att_last_error.argtypes = [P]
@@ -281,6 +282,10 @@ class Context:
     def set_pose_order(self, enabled):
         """Spatial pose order of the single-pair adjoint (default on)."""
         self._chk(self.L.att_set_pose_order(self.h, 1 if enabled else 0))
+
+    def set_modes_waves(self, waves):
+        """Waves per pose of the flexible forward kernel (0: by ligand size)."""
+        self._chk(self.L.att_set_modes_waves(self.h, waves))
 
     # -- uploads ---------------------------------------------------------
     def upload_protein(self, body):

@@ -101,6 +101,12 @@ att_status att_ctx_create(int hip_device, att_ctx** out);
  * L2 holds the grid lines its poses share.  Per-pose results do not depend on
  * it; the parameter-gradient sum runs in a different (still fixed) order. */
 att_status att_set_pose_order(att_ctx* ctx, int enabled);
+/* Waves per pose of the flexible forward kernel (att_score_modes[_dev]):
+ * 1, 2, 4 or 8, or 0 (default: 1 below 384 ligand beads, else 4).  Energy, forces,
+ * torques and ligand mode gradients do not depend on it (bead-order sums);
+ * the receptor mode gradients are summed per wave, then over the waves in
+ * order, so their last bits do. */
+att_status att_set_modes_waves(att_ctx* ctx, int waves);
 void att_ctx_destroy(att_ctx* ctx);
 const char* att_last_error(const att_ctx* ctx);
 
