@@ -59,7 +59,13 @@ struct PGrad {
     const att_pair_params* tab;   // the LDS copy of the pair table
     unsigned T;
     float w;                      // upstream weight of the current pose
+    unsigned* key;                // kPgRecords: per (slot, bead) grid-cell record
+    float4* pos;
+    unsigned cellsIn;             //   key = (cell, inner cells first) * T + type - 1
+    unsigned invalid;             //   key of a bead without a grid term
 };
+
+
 
 // Basis channels read per step of the parameter-gradient loop: the loads of
 // kBasisStep channels are issued together (64 B each) and their LDS updates
@@ -69,9 +75,11 @@ constexpr unsigned kBasisStep = 2;
 
 // Grid term of one ligand bead on one grid: energy (the forward's per-bead
 // value), gradient, and the basis-grid parameter derivatives.
+template <int PG>
 __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, float px, float py,
                                               float pz, unsigned mtype, unsigned type, float charge,
-                                              float3& G, const PGrad& pg) {
+                                              float3& G, const PGrad& pg, unsigned cellBase,
+                                              size_t rec) {
     const unsigned ix = grid_index(px, g.lox, g.dVox_inv, g.nx);
     const unsigned iy = grid_index(py, g.loy, g.dVox_inv, g.ny);
     const unsigned iz = grid_index(pz, g.loz, g.dVox_inv, g.nz);
@@ -90,7 +98,16 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
     // The grid of mapped type mtype holds the bead type's column of the pair
     // table (the grid alphabet maps type -> mtype), so the derivatives go to
     // the entries (a, type).
-    if (pg.s && b.oct && type - 1u < pg.T) {
+    if (PG == kPgRecords) {
+        if (type - 1u < pg.T) {
+            const unsigned o = (iz * (g.ny - 1) + iy) * (g.nx - 1) + ix;
+            pg.key[rec] = (cellBase + o) * pg.T + (type - 1);
+            pg.pos[rec] = make_float4(px, py, pz, pg.w);
+        } else {
+            pg.key[rec] = pg.invalid;
+        }
+    }
+    if (PG == kPgFused && b.oct && type - 1u < pg.T) {
         const float mnx = (float)(int)ix * g.dVox + g.lox;
         const float mny = (float)(int)iy * g.dVox + g.loy;
         const float mnz = (float)(int)iz * g.dVox + g.loz;
@@ -132,14 +149,22 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
 // (model/interpolation.h:56-102).  Each lane selects its grid's geometry and
 // runs the one lookup path, so a wave whose beads straddle the inner grid's
 // boundary does not walk the lookup (and the basis loop) twice.
+template <int PG>
 __device__ __forceinline__ float grid_adj(const DGrid& inner, const DGrid& outer,
                                           const DBasis& bin, const DBasis& bout, const DPar& par,
                                           float px, float py, float pz, unsigned mtype,
-                                          unsigned type, float charge, float3& G, const PGrad& pg) {
+                                          unsigned type, float charge, float3& G, const PGrad& pg,
+                                          size_t rec) {
     G = make_float3(0.f, 0.f, 0.f);
-    if (mtype == 0) return 0.f;
+    if (mtype == 0) {
+        if (PG == kPgRecords) pg.key[rec] = pg.invalid;
+        return 0.f;
+    }
     const bool in = !grid_out(inner, px, py, pz);
-    if (!in && grid_out(outer, px, py, pz)) return 0.f;
+    if (!in && grid_out(outer, px, py, pz)) {
+        if (PG == kPgRecords) pg.key[rec] = pg.invalid;
+        return 0.f;
+    }
     DGrid g{};
     g.oct = in ? inner.oct : outer.oct;
     g.nx = in ? inner.nx : outer.nx;
@@ -154,7 +179,7 @@ __device__ __forceinline__ float grid_adj(const DGrid& inner, const DGrid& outer
     DBasis b{};
     b.oct = in ? bin.oct : bout.oct;
     b.nb = in ? bin.nb : bout.nb;
-    return grid_adj_one(g, b, px, py, pz, mtype, type, charge, G, pg);
+    return grid_adj_one<PG>(g, b, px, py, pz, mtype, type, charge, G, pg, in ? 0u : pg.cellsIn, rec);
 }
 
 // LJ derivatives with respect to (rc, ac, emin) at squared distance dr2.
@@ -178,7 +203,7 @@ __device__ __forceinline__ void lj_dparam(float dr2, float dr2_inv, const att_pa
 //        through the receptor's deformation);
 //  the parameter derivatives into this wave's LDS slice.
 // Receptor beads are deformed from their padded mode row (one cache line).
-template <int MM>
+template <int MM, int PG>
 __device__ __forceinline__ void adj_walk(const PairDesc& P, const att_pair_params* tab,
                                          const PGrad& pg, float px, float py, float pz, float q,
                                          unsigned lt, uint2 d, const float* ampR, float* gR,
@@ -247,7 +272,7 @@ __device__ __forceinline__ void adj_walk(const PairDesc& P, const att_pair_param
             for (unsigned m = 0; m < (unsigned)MA; ++m)
                 if (m < Mr) gR[m] += Fx * mt[3 * m] + Fy * mt[3 * m + 1] + Fz * mt[3 * m + 2];
         }
-        if (pg.s) {
+        if (PG != kPgNone) {
             float drc, dac, dem, drcP, dacP, demP;
             lj_dparam(dr2, dr2_inv, pp, par.shape, drc, dac, dem);
             lj_dparam(nl.P2, nl.P2_inv, pp, par.shape, drcP, dacP, demP);
@@ -283,9 +308,10 @@ struct PoseLDS {
 //      energy is the reference's bits), and the chain rule through p = R x~ + t
 //      by wave 0, bead k on lane k % 64, lane sums reduced by a butterfly.
 // Every sum has a fixed order: results do not depend on timing.
-template <int MM, int WAVES>
+template <int MM, int WAVES, int PG>
 __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, PoseLDS<WAVES>& S,
-                                             const att_pair_params* tab, float* s_pg) {
+                                             const att_pair_params* tab, float* s_pg,
+                                             size_t recBase) {
     const DProt& lig = P.lig;
     const DPar& par = P.par;
     const unsigned tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -297,7 +323,8 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
     const float* d = P.dofs + (size_t)P.dofStride * pose;
     const float tx = d[0], ty = d[1], tz = d[2];
     const float wp = P.wts ? P.wts[pose] : 1.f;
-    const PGrad pg{s_pg ? s_pg + wave * 3 * par.T * par.T : nullptr, tab, par.T, wp};
+    const PGrad pg{PG != kPgNone ? s_pg + wave * 3 * par.T * par.T : nullptr, tab, par.T, wp,
+                   P.recKey, P.recPos, P.recCellsIn, P.recInvalid};
     float ampR[MA], ampL[MA];
 #pragma unroll
     for (int m = 0; m < MA; ++m) {
@@ -337,8 +364,9 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
             float4 x;
             const V3 p = place(sc + k, x);
             float3 G;
-            const float eg = grid_adj(P.inner, P.outer, P.bin, P.bout, par, p.x, p.y, p.z,
-                                      lig.mtype[sc + k], lig.type[sc + k], x.w, G, pg);
+            const float eg = grid_adj<PG>(P.inner, P.outer, P.bin, P.bout, par, p.x, p.y, p.z,
+                                          lig.mtype[sc + k], lig.type[sc + k], x.w, G, pg,
+                                          recBase + sc + k);
             S.g[k] = make_float4(G.x, G.y, G.z, eg);
             S.nlc[k] = nl_cell(P.nl, p.x, p.y, p.z, lig.type[sc + k]);
         }
@@ -354,7 +382,7 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                 const V3 p = place(sc + k, x);
                 float eAcc;
                 float3 gAcc;
-                adj_walk<MM>(P, tab, pg, p.x, p.y, p.z, x.w, lig.type[sc + k], S.nlc[k], ampR, gR,
+                adj_walk<MM, PG>(P, tab, pg, p.x, p.y, p.z, x.w, lig.type[sc + k], S.nlc[k], ampR, gR,
                              eAcc, gAcc);
                 float4 g = S.g[k];
                 g.x = g.x - gAcc.x; g.y = g.y - gAcc.y; g.z = g.z - gAcc.z;
@@ -442,7 +470,7 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
 template <int MM>
 constexpr int adjoint_waves_per_simd() { return MM >= 10 ? 3 : 4; }
 
-template <int MM, int WAVES>
+template <int MM, int WAVES, int PG>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjoint_waves_per_simd<MM>()))) void k_adjoint(
     PairDesc P, float* __restrict__ pgPartial) {
     extern __shared__ att_pair_params s_dyn[];
@@ -452,9 +480,9 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
     // Parameter derivatives: one LDS slice per wave, so that every slice is
     // updated by one wave only, in program order; the slices are added in
     // wave order at the end.  Runs therefore give the same bits.
-    float* s_pg = pgPartial ? reinterpret_cast<float*>(dyn + adjoint_pg_offset(P.tabSlots)) : nullptr;
+    float* s_pg = PG != kPgNone ? reinterpret_cast<float*>(dyn + adjoint_pg_offset(P.tabSlots)) : nullptr;
     const unsigned npg = 3 * P.par.T * P.par.T;
-    if (s_pg)
+    if (PG != kPgNone)
         for (unsigned k = threadIdx.x; k < npg * WAVES; k += blockDim.x) s_pg[k] = 0.f;
     const unsigned n = P.nDev ? min(P.nDev[0], P.n) : P.n;
     // With a spatial order (and at least 8 workgroups), workgroup b (on XCD
@@ -468,9 +496,10 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
         first = (unsigned)(((unsigned long long)n * x) >> 3) + (blockIdx.x >> 3);
         end = (unsigned)(((unsigned long long)n * (x + 1)) >> 3);
     }
+    // grid-cell records of slot i (kPgRecords) start at i * ligand beads
     for (unsigned i = first; i < end; i += step)
-        adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, tab, s_pg);
-    if (s_pg) {
+        adjoint_pose<MM, WAVES, PG>(P, P.order ? P.order[i] : i, S, tab, s_pg, (size_t)i * P.lig.n);
+    if (PG != kPgNone) {
         __syncthreads();
         for (unsigned k = threadIdx.x; k < npg; k += blockDim.x) {
             float v = s_pg[k];
@@ -511,7 +540,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
                 cur = tile.pair;
             }
             for (unsigned i = tile.first; i < tile.first + tile.count; ++i)
-                adjoint_pose<MM, WAVES>(P, P.order ? P.order[i] : i, S, tab, nullptr);
+                adjoint_pose<MM, WAVES, kPgNone>(P, P.order ? P.order[i] : i, S, tab, nullptr, 0);
         }
     }
 }
@@ -605,25 +634,31 @@ size_t adjoint_static_lds(int waves) {
 }
 
 namespace {
-template <int MM>
+template <int MM, int PG>
 const void* adjoint_kernel(int waves, bool batch) {
     switch (waves) {
-        case 1: return batch ? (const void*)k_adjoint_batch<MM, 1> : (const void*)k_adjoint<MM, 1>;
-        case 2: return batch ? (const void*)k_adjoint_batch<MM, 2> : (const void*)k_adjoint<MM, 2>;
-        default: return batch ? (const void*)k_adjoint_batch<MM, 4> : (const void*)k_adjoint<MM, 4>;
+        case 1: return batch ? (const void*)k_adjoint_batch<MM, 1> : (const void*)k_adjoint<MM, 1, PG>;
+        case 2: return batch ? (const void*)k_adjoint_batch<MM, 2> : (const void*)k_adjoint<MM, 2, PG>;
+        default: return batch ? (const void*)k_adjoint_batch<MM, 4> : (const void*)k_adjoint<MM, 4, PG>;
     }
 }
-const void* adjoint_kernel_for(unsigned mm, int waves, bool batch) {
-    if (mm == 0) return adjoint_kernel<0>(waves, batch);
-    if (mm <= 5) return adjoint_kernel<5>(waves, batch);
-    return adjoint_kernel<ATT_MODES_MAX>(waves, batch);
+template <int PG>
+const void* adjoint_kernel_pg(unsigned mm, int waves, bool batch) {
+    if (mm == 0) return adjoint_kernel<0, PG>(waves, batch);
+    if (mm <= 5) return adjoint_kernel<5, PG>(waves, batch);
+    return adjoint_kernel<ATT_MODES_MAX, PG>(waves, batch);
+}
+const void* adjoint_kernel_for(unsigned mm, int waves, bool batch, int pg = kPgNone) {
+    if (batch || pg == kPgNone) return adjoint_kernel_pg<kPgNone>(mm, waves, batch);
+    if (pg == kPgFused) return adjoint_kernel_pg<kPgFused>(mm, waves, false);
+    return adjoint_kernel_pg<kPgRecords>(mm, waves, false);
 }
 }  // namespace
 
-unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds) {
+unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds, int pg) {
     int b = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &b, adjoint_kernel_for(mm, waves, batch), 64 * waves, lds);
+        &b, adjoint_kernel_for(mm, waves, batch, pg), 64 * waves, lds);
     return e == hipSuccess && b > 0 ? (unsigned)b : 1u;
 }
 
@@ -631,8 +666,9 @@ hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsig
                           int waves, size_t lds, hipStream_t s) {
     if (P.n == 0) return hipSuccess;
     const unsigned mm = modes ? (P.rec.M > P.lig.M ? P.rec.M : P.lig.M) : 0;
+    const int pg = !pgPartial ? kPgNone : P.recKey ? kPgRecords : kPgFused;
     void* args[] = {const_cast<PairDesc*>(&P), &pgPartial};
-    return hipLaunchKernel(adjoint_kernel_for(mm, waves, false), dim3(blocks), dim3(64 * waves),
+    return hipLaunchKernel(adjoint_kernel_for(mm, waves, false, pg), dim3(blocks), dim3(64 * waves),
                            args, lds, s);
 }
 

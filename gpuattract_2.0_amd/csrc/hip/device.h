@@ -526,6 +526,12 @@ struct PairDesc {
     unsigned dofStride, outStride, n, tabSlots;
     const unsigned* nDev;  // when set, the number of poses is read here at launch (<= n)
     const unsigned* order; // when set, a spatial permutation of the poses (XCD ranges)
+    // Grid part of the parameter gradient as records (when set): slot i's bead
+    // k writes recKey/recPos[i * lig.n + k]; keys (cell * T + type - 1, inner
+    // cells first) below recInvalid, recInvalid for beads without a grid term.
+    unsigned* recKey;
+    float4* recPos;
+    unsigned recCellsIn, recInvalid;
 };
 
 // Slot of workgroup b among nb in a non-persistent launch such that the

@@ -46,7 +46,37 @@ size_t adjoint_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, bo
 // Static LDS of the adjoint kernels with `waves` waves per pose, and how many
 // of their workgroups one CU holds with `lds` bytes of dynamic LDS.
 size_t adjoint_static_lds(int waves);
-unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds);
+unsigned adjoint_blocks_per_cu(unsigned mm, int waves, bool batch, size_t lds, int pg = 0);
+
+// How k_adjoint takes the grid part of the parameter gradient: not at all (no
+// parameter gradient), from the basis grids at every bead (kPgFused), or as
+// one record per bead {grid cell and bead type; position and weight} that the
+// cell pass (pgcells.hip) contracts with the basis grids once per distinct
+// (cell, type) after sorting (kPgRecords; PairDesc.recKey set).
+constexpr int kPgNone = 0, kPgFused = 1, kPgRecords = 2;
+
+// Scratch of the cell pass for `records` records: the adjoint's records
+// (keys, then positions) at the front, sort and reduction buffers after.
+struct PgCellBufs {
+    unsigned* key;       // records as written by k_adjoint
+    float4* pos;
+    unsigned* keySorted;
+    float4* posSorted;
+    unsigned* cellKey;   // runs of equal keys (per group of 64 sorted records), in order
+    float* cellW;        // their summed corner weights, 8 per run
+    unsigned* groupRuns; // runs per group of 64 records
+    unsigned* groupOffset;  // and their exclusive prefix sums
+    void* temp;
+    size_t tempBytes;
+};
+size_t pg_cells_bytes(size_t records, unsigned endBit);
+PgCellBufs pg_cells_layout(void* scratch, size_t records, unsigned endBit);
+// Cell pass over the records of P's k_adjoint launch: sort by key, sum the
+// corner weights of each run of equal keys (fixed order), then `blocks`
+// one-wave workgroups contract every run with its cell's basis values and
+// write one parameter row each to rows[0 .. blocks).
+hipError_t launch_pg_cells(const PairDesc& P, const PgCellBufs& B, size_t records, unsigned endBit,
+                           float* rows, unsigned blocks, hipStream_t s);
 
 // One pair: persistent grid of `blocks` workgroups of 64*waves threads.
 hipError_t launch_adjoint(const PairDesc& P, bool modes, float* pgPartial, unsigned blocks,

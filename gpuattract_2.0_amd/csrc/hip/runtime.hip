@@ -77,6 +77,8 @@ struct att_ctx {
     unsigned numCU = 256;
     void* d_pg = nullptr;  // per-workgroup parameter-gradient rows of the adjoint
     size_t d_pg_cap = 0;
+    void* d_rec = nullptr;  // grid-cell records of the parameter gradient and their cell pass
+    size_t d_rec_cap = 0;
     void* d_w = nullptr;   // staging of host-buffer adjoint calls
     size_t d_w_cap = 0;
     void* d_g = nullptr;
@@ -85,7 +87,7 @@ struct att_ctx {
     size_t d_ord_cap = 0;
     bool poseOrder = true;
     int modesWaves = 0;  // waves per pose of the flexible forward kernel, 0: by ligand size
-    // Recorded after the last kernel that used d_pg or d_ord: a call on
+    // Recorded after the last kernel that used d_pg, d_rec or d_ord: a call on
     // another stream waits for it before reusing them.
     hipEvent_t scratchEv = nullptr;
     // Device address of this GPU's `scratch` (att_score_adjoint_dev_noalloc).
@@ -288,6 +290,11 @@ namespace {
 // two of 1, 2, 4).  Both paths use it, so a point scored by either gives the
 // same bits.
 int pair_waves(unsigned ligN) { return ligN < 384 ? 1 : 2; }
+// Parameter-gradient records per call up to which the cell pass takes the
+// grid part (76 B of scratch per record: 10 GB at the cap), and the
+// workgroups of that pass (one parameter row each).
+constexpr size_t kPgRecordCap = (size_t)1 << 27;
+constexpr unsigned kPgCellBlocks = 2048;
 // Waves per pose of the flexible forward kernel (k_modes): with 384 ligand
 // beads or more, four (3000/800 beads, 5 + 5 modes: 100,000 poses 3% slower
 // than with two, a call of 100 poses 1.6x faster).  A function of the pair
@@ -368,6 +375,7 @@ void att_ctx_destroy(att_ctx* c) {
     if (c->d_in) (void)hipFree(c->d_in);
     if (c->d_out) (void)hipFree(c->d_out);
     if (c->d_pg) (void)hipFree(c->d_pg);
+    if (c->d_rec) (void)hipFree(c->d_rec);
     if (c->d_ord) (void)hipFree(c->d_ord);
     if (c->d_w) (void)hipFree(c->d_w);
     if (c->d_g) (void)hipFree(c->d_g);
@@ -727,13 +735,29 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
         return fail(c, ATT_ERR_UNSUPPORTED, "ligand too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const unsigned cols = 3 * p->d.T * p->d.T;
+    // The grid part of the parameter gradient through grid-cell records and
+    // the cell pass, up to kPgRecordCap records (bead x pose) per call; larger
+    // calls take it from the basis grids at every bead.
+    const size_t records = pgrad ? n * (size_t)lig->d.n : 0;
+    const size_t cellsIn = (size_t)(g->inner.nx - 1) * (g->inner.ny - 1) * (g->inner.nz - 1);
+    const size_t cellsOut = (size_t)(g->outer.nx - 1) * (g->outer.ny - 1) * (g->outer.nz - 1);
+    const size_t invalid = (cellsIn + cellsOut) * p->d.T;
+    const bool useRec = records > 0 && records <= kPgRecordCap && invalid < 0xFFFFFFFFull &&
+                        sizeof(float) * cols <= c->maxLdsPerBlock;
+    const int pgMode = !pgrad ? kPgNone : useRec ? kPgRecords : kPgFused;
     // persistent grid: as many workgroups as the device holds at once
     const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
-    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
+    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn, pgMode);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
-    // per-workgroup rows, then the reduction's group sums (8-byte aligned)
-    const size_t rowBytes = (sizeof(float) * (size_t)blocks * cols + 255) & ~(size_t)255;
+    const unsigned cellBlocks = useRec ? kPgCellBlocks : 0;
+    // per-workgroup rows (the adjoint's, then the cell pass's), then the
+    // reduction's group sums (8-byte aligned)
+    const size_t rowBytes = (sizeof(float) * ((size_t)blocks + cellBlocks) * cols + 255) & ~(size_t)255;
     if (pgrad && (st = grow(c, &c->d_pg, &c->d_pg_cap, rowBytes + pgrad_group_bytes(cols))) != ATT_OK)
+        return st;
+    unsigned endBit = 0;
+    while (endBit < 32 && (invalid >> endBit) != 0) ++endBit;
+    if (useRec && (st = grow(c, &c->d_rec, &c->d_rec_cap, pg_cells_bytes(records, endBit))) != ATT_OK)
         return st;
     if (n == 0) {
         if (pgrad) ATT_HIP(c, launch_fill_zero(d_pgrad, cols, s));
@@ -747,14 +771,24 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
     P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
     P.n = (unsigned)n; P.tabSlots = slots;
+    PgCellBufs B{};
+    if (useRec) {
+        B = pg_cells_layout(c->d_rec, records, endBit);
+        P.recKey = B.key;
+        P.recPos = B.pos;
+        P.recCellsIn = (unsigned)cellsIn;
+        P.recInvalid = (unsigned)invalid;
+    }
     if (blocks >= 8 && (st = pose_order(c, d_dofs, P.dofStride, n, 4 * (size_t)blocks, s, &P.order)) != ATT_OK)
         return st;
     const bool scratch = pgrad || P.order;
     if (pgrad && !P.order) ATT_HIP(c, hipStreamWaitEvent(s, c->scratchEv, 0));
-    ATT_HIP(c, launch_adjoint(P, modes != 0, pgrad ? static_cast<float*>(c->d_pg) : nullptr, blocks,
-                              kWaves, dyn, s));
+    float* rows = pgrad ? static_cast<float*>(c->d_pg) : nullptr;
+    ATT_HIP(c, launch_adjoint(P, modes != 0, rows, blocks, kWaves, dyn, s));
+    if (useRec)
+        ATT_HIP(c, launch_pg_cells(P, B, records, endBit, rows + (size_t)blocks * cols, cellBlocks, s));
     if (pgrad)
-        ATT_HIP(c, launch_pgrad_reduce(static_cast<const float*>(c->d_pg), blocks, cols,
+        ATT_HIP(c, launch_pgrad_reduce(rows, blocks + cellBlocks, cols,
                                        reinterpret_cast<double*>(static_cast<char*>(c->d_pg) + rowBytes),
                                        d_pgrad, s));
     if (scratch) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
@@ -789,7 +823,8 @@ att_status att_score_adjoint_dev_noalloc(att_ctx* c, att_handle hr, att_handle h
         return fail(c, ATT_ERR_UNSUPPORTED, "pair table too large for the LDS of the adjoint kernel");
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
     const unsigned mm = modes ? std::max(rec->d.M, lig->d.M) : 0;
-    const size_t resident = (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn);
+    const size_t resident =
+        (size_t)c->numCU * adjoint_blocks_per_cu(mm, kWaves, false, dyn, pgrad ? kPgFused : kPgNone);
     const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(n, resident));
     if (pgrad && blocks > kNoAllocBlocks)
         return fail(c, ATT_ERR_UNSUPPORTED, "more resident workgroups than the no-allocation scratch holds");

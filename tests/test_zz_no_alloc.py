@@ -1,8 +1,12 @@
 """att_score_adjoint_dev_noalloc (needs a GPU): the allocation-free variant of
 att_score_adjoint_dev returns the same bits as that entry point run in input
-pose order, on two calls in a row (the second with fewer poses, reusing the
-per-GPU scratch), with and without parameter gradients, for one and two waves
-per pose; tables beyond its scratch are refused."""
+pose order for the energies and the DOF and mode gradients, on two calls in a
+row (the second with fewer poses, reusing the per-GPU scratch), with and
+without parameter gradients, for one and two waves per pose.  The parameter
+gradient's grid part is summed in another fixed order (at every bead from the
+basis grids, where att_score_adjoint_dev sums per grid cell after sorting
+its records), so it agrees to FP32 rounding (1e-5 of its largest entry) and
+repeats bit for bit.  Tables beyond its scratch are refused."""
 import numpy as np
 import pytest
 
@@ -39,8 +43,14 @@ def test_noalloc_adjoint_equals_allocating_entry_point(W, n_lig, modes):
             a, b = (ctx.to_host(o, (n, ow)) for o in outs)
             pa, pb = (ctx.to_host(p, (3 * T * T,)) for p in pgs)
             assert np.array_equal(_bits(a), _bits(b)), n
-            assert np.array_equal(_bits(pa), _bits(pb)), n
-            assert np.abs(pa).max() > 0
+            scale = np.abs(pa).max()
+            assert scale > 0
+            err = np.abs(pa - pb).max() / scale
+            print(f"{n} poses: parameter gradients agree to {err:.1e} of the largest entry")
+            assert err <= 1e-5, n
+            ctx.adjoint_dev_noalloc(hs, d_in.ptr, modes, n, d_w.ptr, outs[1].ptr, pgs[1].ptr)
+            ctx.sync()
+            assert np.array_equal(_bits(ctx.to_host(pgs[1], (3 * T * T,))), _bits(pb)), n
             # without parameter gradients
             ctx.adjoint_dev_noalloc(hs, d_in.ptr, modes, n, None, outs[1].ptr)
             ctx.sync()
