@@ -207,7 +207,7 @@ template <int MM, int PG>
 __device__ __forceinline__ void adj_walk(const PairDesc& P, const att_pair_params* tab,
                                          const PGrad& pg, float px, float py, float pz, float q,
                                          unsigned lt, uint2 d, const float* ampR, float* gR,
-                                         float& eAcc, float3& gAcc) {
+                                         float& eAcc, float3& gAcc, unsigned& inside) {
     constexpr int MA = MM > 0 ? MM : 1;
     const DNL& nl = P.nl;
     const DPar& par = P.par;
@@ -247,6 +247,7 @@ __device__ __forceinline__ void adj_walk(const PairDesc& P, const att_pair_param
         const float ddx = px - yx, ddy = py - yy, ddz = pz - yz;
         const float dr2 = ddx * ddx + ddy * ddy + ddz * ddz;
         if (!(dr2 <= nl.P2) || rt - 1u >= par.T) continue;
+        ++inside;
         const float dr2_inv = (float)(1.0 / (double)dr2);
         const float dx = ddx * dr2_inv, dy = ddy * dr2_inv, dz = ddz * dr2_inv;
         const unsigned kk = par.T * (rt - 1) + lt1;
@@ -294,6 +295,7 @@ struct PoseLDS {
     unsigned short q[kSuper];
     unsigned cc[queue_counters(kSuper)];
     float red[WAVES][ATT_MODES_MAX];
+    unsigned cnt[WAVES][2];
     float e;
 };
 
@@ -342,6 +344,7 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
 #pragma unroll
     for (int m = 0; m < MA; ++m) { gR[m] = 0.f; gL[m] = 0.f; }
     float esum = 0.f;
+    unsigned listed = 0, inside = 0;  // this lane's pair counts (P.counts != nullptr)
 
     auto deformed = [&](unsigned i, float4& x) __attribute__((always_inline)) {
         x = lig.pos[i];
@@ -383,7 +386,8 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                 float eAcc;
                 float3 gAcc;
                 adj_walk<MM, PG>(P, tab, pg, p.x, p.y, p.z, x.w, lig.type[sc + k], S.nlc[k], ampR, gR,
-                             eAcc, gAcc);
+                                 eAcc, gAcc, inside);
+                listed += S.nlc[k].x;
                 float4 g = S.g[k];
                 g.x = g.x - gAcc.x; g.y = g.y - gAcc.y; g.z = g.z - gAcc.z;
                 g.w = g.w + eAcc;  // outLig_E[i] += eAcc
@@ -440,7 +444,17 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
         for (int m = 0; m < ATT_MODES_MAX; ++m) S.red[wave][m] = m < MA ? gR[m < MA ? m : 0] : 0.f;
     }
     if (tid == eThread) S.e = esum;
+    if (P.counts) {
+        listed = wave_sum_u(listed);
+        inside = wave_sum_u(inside);
+        if (lane == 0) { S.cnt[wave][0] = listed; S.cnt[wave][1] = inside; }
+    }
     __syncthreads();
+    if (P.counts && tid == 0) {
+        uint2 c = make_uint2(0u, 0u);
+        for (unsigned w = 0; w < (unsigned)WAVES; ++w) { c.x += S.cnt[w][0]; c.y += S.cnt[w][1]; }
+        P.counts[pose] = c;
+    }
     if (tid == 0) {
         float da[3];
         torque_reduce(t[0], t[1], t[2], t[3], t[4], t[5], acc + 3, da);

@@ -477,6 +477,13 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// The same over unsigned counters (exact: any order gives the same sum).
+__device__ __forceinline__ unsigned wave_sum_u(unsigned v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
 // reducePotForce clamp (model/reduction.h:60-72); x *= 0.01 is a double multiply.
 __device__ __forceinline__ void force_clamp(float& x, float& y, float& z) {
 #pragma unroll
@@ -532,6 +539,9 @@ struct PairDesc {
     unsigned* recKey;
     float4* recPos;
     unsigned recCellsIn, recInvalid;
+    // when set, per pose {listed neighbours of the beads walked, pairs inside
+    // the plateau}
+    uint2* counts;
 };
 
 // Slot of workgroup b among nb in a non-persistent launch such that the

@@ -10,9 +10,11 @@ namespace att {
 
 // `order`: a spatial permutation of the poses (or null); workgroups of one
 // XCD then take a contiguous range of it (xcd_slot).
+// `counts` (or null): per pose {listed neighbours of the beads walked, pairs
+// inside the plateau}.
 hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, const DGrid& outer,
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
-                        const unsigned* order, hipStream_t s);
+                        const unsigned* order, uint2* counts, hipStream_t s);
 
 size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsigned* tabSlots);
 
@@ -20,7 +22,7 @@ size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsi
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
-                        unsigned n, const unsigned* order, int waves, hipStream_t s);
+                        unsigned n, const unsigned* order, int waves, uint2* counts, hipStream_t s);
 
 hipError_t launch_debug_rigid(const DProt& rec, const DProt& lig, const DGrid& inner,
                               const DGrid& outer, const DNL& nl, const DPar& par,

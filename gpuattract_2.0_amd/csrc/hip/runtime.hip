@@ -87,6 +87,8 @@ struct att_ctx {
     size_t d_ord_cap = 0;
     bool poseOrder = true;
     int modesWaves = 0;  // waves per pose of the flexible forward kernel, 0: by ligand size
+    uint2* pairCounts = nullptr;  // att_debug_pair_counts: per-pose counts, capacity in poses
+    size_t pairCountsCap = 0;
     // Recorded after the last kernel that used d_pg, d_rec or d_ord: a call on
     // another stream waits for it before reusing them.
     hipEvent_t scratchEv = nullptr;
@@ -327,6 +329,23 @@ const char* att_build_info(void) {
 att_status att_set_pose_order(att_ctx* c, int enabled) {
     if (!c) return ATT_ERR_INVALID;
     c->poseOrder = enabled != 0;
+    return ATT_OK;
+}
+
+att_status att_debug_pair_counts(att_ctx* c, void* d_counts, size_t capacity) {
+    if (!c) return ATT_ERR_INVALID;
+    c->pairCounts = static_cast<uint2*>(d_counts);
+    c->pairCountsCap = d_counts ? capacity : 0;
+    return ATT_OK;
+}
+
+// The per-pose pair-count buffer for a call of n poses (null when off).
+static att_status pair_counts(att_ctx* c, size_t n, uint2** out) {
+    *out = nullptr;
+    if (!c->pairCounts) return ATT_OK;
+    if (n > c->pairCountsCap)
+        return fail(c, ATT_ERR_INVALID, "pair-count buffer holds fewer poses than the call");
+    *out = c->pairCounts;
     return ATT_OK;
 }
 
@@ -594,12 +613,14 @@ att_status att_score_rigid_dev(att_ctx* c, att_handle hr, att_handle hl, att_han
     if ((st = resolve(c, hr, hl, hg, hp, &rec, &lig, &g, &p)) != ATT_OK) return st;
     if ((st = check_pair(c, rec, lig, g, p)) != ATT_OK) return st;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    uint2* counts;
+    if ((st = pair_counts(c, n, &counts)) != ATT_OK) return st;
     const unsigned* order = nullptr;
     if ((st = pose_order(c, reinterpret_cast<const float*>(d_dofs), 6, n, kForwardOrderMin, s, &order)) != ATT_OK)
         return st;
     ATT_HIP(c, launch_rigid(rec->d, lig->d, g->inner, g->outer, g->nl, p->d,
                             reinterpret_cast<const float*>(d_dofs), reinterpret_cast<float*>(d_out),
-                            (unsigned)n, order, s));
+                            (unsigned)n, order, counts, s));
     if (order) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
@@ -648,13 +669,15 @@ att_status att_score_modes_dev(att_ctx* c, att_handle hr, att_handle hl, att_han
     if (!gl) return fail(c, ATT_ERR_HANDLE, "unknown ligand grid handle");
     if ((st = check_modes(c, rec, lig, gr, gl, p)) != ATT_OK) return st;
     hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+    uint2* counts;
+    if ((st = pair_counts(c, n, &counts)) != ATT_OK) return st;
     const unsigned* order = nullptr;
     if ((st = pose_order(c, reinterpret_cast<const float*>(d_dofs), 26, n, kForwardOrderMin, s, &order)) != ATT_OK)
         return st;
     ATT_HIP(c, launch_modes(rec->d, lig->d, gr->inner, gr->outer, gr->nl, gl->inner, gl->outer,
                             p->d, reinterpret_cast<const float*>(d_dofs),
                             reinterpret_cast<float*>(d_out), (unsigned)n, order,
-                            c->modesWaves ? c->modesWaves : modes_waves(lig->d.n), s));
+                            c->modesWaves ? c->modesWaves : modes_waves(lig->d.n), counts, s));
     if (order) ATT_HIP(c, hipEventRecord(c->scratchEv, s));
     return ATT_OK;
 }
@@ -771,6 +794,7 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
     P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
     P.n = (unsigned)n; P.tabSlots = slots;
+    if ((st = pair_counts(c, n, &P.counts)) != ATT_OK) return st;
     PgCellBufs B{};
     if (useRec) {
         B = pg_cells_layout(c->d_rec, records, endBit);
@@ -840,6 +864,7 @@ att_status att_score_adjoint_dev_noalloc(att_ctx* c, att_handle hr, att_handle h
     P.dofs = d_dofs; P.wts = d_w; P.out = d_out;
     P.dofStride = modes ? 26 : 6; P.outStride = modes ? 27 : 7;
     P.n = (unsigned)n; P.tabSlots = slots;
+    if ((st = pair_counts(c, n, &P.counts)) != ATT_OK) return st;
     P.order = nullptr;  // input order: the spatial order needs sort scratch
     // Every workgroup writes its whole row and the reduction every group sum,
     // so the scratch needs no clearing between calls.

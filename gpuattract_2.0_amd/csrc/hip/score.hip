@@ -37,7 +37,8 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
                                                DNL nl, DPar par, const float* __restrict__ dofs,
                                                float* __restrict__ res, unsigned n,
                                                unsigned tabSlots,
-                                               const unsigned* __restrict__ order) {
+                                               const unsigned* __restrict__ order,
+                                               uint2* __restrict__ counts) {
     extern __shared__ __align__(16) att_pair_params s_tab[];
     const att_pair_params* tab = stage_table(par, s_tab);
     __syncthreads();
@@ -65,6 +66,7 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
         return p;
     };
     float red = 0.f;  // the reference's reduction held by this lane (reduce_beads_ref)
+    unsigned listed = 0, inside = 0;  // this lane's pair counts (counts != nullptr)
     const unsigned span = super_chunk(lig.n, kRigidSuper);
     for (unsigned sc = 0; sc < lig.n; sc += span) {
         const unsigned m = min(span, lig.n - sc);
@@ -87,9 +89,12 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
                 const unsigned k = s_q[q0 + lane];
                 float4 x;
                 const V3 p = place(sc + k, x);
+                unsigned np = 0;
                 const float4 a = nl_walk(nl, par, tab, RecPosG{rec.pos}, RecTypeG{rec.type},
                                          RecChargeG{rec.pos}, NoReaction{}, p.x, p.y, p.z,
-                                         lig.type[sc + k], x.w, s_n[k], nullptr);
+                                         lig.type[sc + k], x.w, s_n[k], &np);
+                listed += s_n[k].x;
+                inside += np;
                 float4 f = s_f[k];
                 f.x = f.x + a.x; f.y = f.y + a.y; f.z = f.z + a.z; f.w = f.w + a.w;
                 s_f[k] = f;
@@ -106,6 +111,11 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
         float* r = res + 7 * (size_t)pose;
 #pragma unroll
         for (int q = 0; q < 7; ++q) r[q] = o[q];
+    }
+    if (counts) {
+        listed = wave_sum_u(listed);
+        inside = wave_sum_u(inside);
+        if (lane == 0) counts[pose] = make_uint2(listed, inside);
     }
 }
 
@@ -196,7 +206,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
                                                   DNL nl, DGrid lInner, DGrid lOuter, DPar par,
                                                   const float* __restrict__ dofs,
                                                   float* __restrict__ res, unsigned n,
-                                                  const unsigned* __restrict__ order) {
+                                                  const unsigned* __restrict__ order,
+                                                  uint2* __restrict__ counts) {
     constexpr unsigned kSuper = 256u * W;
     extern __shared__ __align__(16) att_pair_params s_tab[];
     __shared__ float4 s_f[kSuper];
@@ -205,6 +216,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
     __shared__ unsigned s_cc[queue_counters(kSuper)];
     __shared__ float s_red[W][ATT_MODES_MAX];
     __shared__ float s_gl[ATT_MODES_MAX];
+    __shared__ unsigned s_cnt[W][2];
     const att_pair_params* tab = stage_table(par, s_tab);  // visible after phase B's first barrier
     if (blockIdx.x >= n) return;  // uniform per workgroup
     const unsigned pose = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
@@ -275,6 +287,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
     };
     const unsigned Ml = lig.M, nL = lig.n;
     float red = 0.f;  // wave 0: E, F, torque tensor; modeWave: ligand modes
+    unsigned listed = 0, inside = 0;  // this lane's pair counts (counts != nullptr)
     const unsigned span = super_chunk(nL, kSuper);
     for (unsigned sc = 0; sc < nL; sc += span) {
         const unsigned m = min(span, nL - sc);
@@ -295,9 +308,12 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
                 const unsigned k = s_q[q0 + tid];
                 float4 x;
                 const V3 p = place(sc + k, x);
+                unsigned np = 0;
                 const float4 a = nl_walk<1>(nl, par, tab, RecPosDeform<MM>{rpos, rmodesT, rowT, Mr, ampR},
                                          RecTypeG{rec.type}, RecChargeG{rpos}, reaction, p.x, p.y,
-                                         p.z, lig.type[sc + k], x.w, s_n[k], nullptr);
+                                         p.z, lig.type[sc + k], x.w, s_n[k], &np);
+                listed += s_n[k].x;
+                inside += np;
                 float4 f = s_f[k];
                 f.x = f.x + a.x; f.y = f.y + a.y; f.z = f.z + a.z; f.w = f.w + a.w;
                 s_f[k] = f;
@@ -323,7 +339,17 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
         const unsigned ml = W > 1 ? lane : lane - 13u;
         if (ml < ATT_MODES_MAX) s_gl[ml] = red;
     }
+    if (counts) {
+        listed = wave_sum_u(listed);
+        inside = wave_sum_u(inside);
+        if (lane == 0) { s_cnt[wave][0] = listed; s_cnt[wave][1] = inside; }
+    }
     __syncthreads();
+    if (counts && threadIdx.x == 0) {
+        uint2 c = make_uint2(0u, 0u);
+        for (int w = 0; w < W; ++w) { c.x += s_cnt[w][0]; c.y += s_cnt[w][1]; }
+        counts[pose] = c;
+    }
     if (wave == 0) {
         float o[7];
         finish_ref(red, t, o);
@@ -504,14 +530,14 @@ hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s) {
 
 hipError_t launch_rigid(const DProt& rec, const DProt& lig, const DGrid& inner, const DGrid& outer,
                         const DNL& nl, const DPar& par, const float* dofs, float* res, unsigned n,
-                        const unsigned* order, hipStream_t s) {
+                        const unsigned* order, uint2* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const unsigned posesPerBlock = 4;
     const unsigned blocks = (n + posesPerBlock - 1) / posesPerBlock;
     const unsigned slots = par.T <= kMaxTypesLDS ? par.T * par.T : 0;
     const size_t lds = rigid_wave_offset(slots) + posesPerBlock * kRigidWaveBytes;
     hipLaunchKernelGGL(k_rigid, dim3(blocks), dim3(256), lds, s, rec, lig, inner, outer, nl, par,
-                       dofs, res, n, slots, order);
+                       dofs, res, n, slots, order, counts);
     return hipGetLastError();
 }
 
@@ -527,13 +553,13 @@ size_t modes_lds_bytes(const DProt& rec, const DProt& lig, const DPar& par, unsi
 hipError_t launch_modes(const DProt& rec, const DProt& lig, const DGrid& rInner,
                         const DGrid& rOuter, const DNL& nl, const DGrid& lInner,
                         const DGrid& lOuter, const DPar& par, const float* dofs, float* res,
-                        unsigned n, const unsigned* order, int waves, hipStream_t s) {
+                        unsigned n, const unsigned* order, int waves, uint2* counts, hipStream_t s) {
     if (n == 0) return hipSuccess;
     const size_t lds = modes_lds_bytes(rec, lig, par, nullptr);
     const unsigned mm = rec.M > lig.M ? rec.M : lig.M;
 #define ATT_LAUNCH_MODES(MMV, WV)                                                                 \
     hipLaunchKernelGGL((k_modes<MMV, WV>), dim3(n), dim3(64 * WV), lds, s, rec, lig, rInner, rOuter, \
-                       nl, lInner, lOuter, par, dofs, res, n, order)
+                       nl, lInner, lOuter, par, dofs, res, n, order, counts)
 #define ATT_LAUNCH_MODES_W(MMV)                                                                  \
     switch (waves) {                                                                              \
     case 8: ATT_LAUNCH_MODES(MMV, 8); break;                                                      \

@@ -3,7 +3,10 @@
 // One engine thread owns the device context.  Each loop pass it
 //   1. admits queued jobs in arrival order while their footprint fits the
 //      budget (evicting least recently used receptors and ligands that no
-//      running job uses), uploads what they need, starts their minimizers;
+//      running job uses), uploads what they need, starts their minimizers; a
+//      job that has to wait for memory does not hold back later jobs that fit
+//      now, up to kMaxBypass of them per waiting job, after which admission
+//      waits for it (so a large job is not starved by a stream of small ones);
 //   2. advances every running job whose last chunk of rounds has completed
 //      (event query, no blocking): issues the next chunk on the job's stream,
 //      or copies the results back and releases the job's memory.
@@ -91,6 +94,7 @@ struct Job {
     bool finishing = false, chunkPending = false;
     int rounds = 0;
     Clock::time_point tSubmit, tAdmit, tDone;
+    uint64_t admitSeq = 0;  // 1 for the first job admitted, 2 for the next, ...
     bool gridCached = false;
     void* stream = nullptr;
     void* ev = nullptr;
@@ -161,6 +165,9 @@ bool protein_ok(const att_protein_desc* d) {
 
 }  // namespace
 
+// Later jobs admitted past one job that waits for memory, at most.
+constexpr unsigned kMaxBypass = 32;
+
 struct att_server {
     att_ctx* ctx = nullptr;
     size_t budget = 0, used = 0, peak = 0;
@@ -173,6 +180,9 @@ struct att_server {
     std::map<uint64_t, std::unique_ptr<Params>> params;
     std::map<uint64_t, std::unique_ptr<Job>> jobs;
     std::deque<Job*> queue;
+    uint64_t admitted = 0;
+    const Job* waiting = nullptr;  // engine thread only: the oldest job waiting for memory
+    unsigned bypassed = 0;         //   and how many later jobs were admitted past it
     std::vector<Job*> running;   // engine thread only
     std::vector<void*> streams;  // engine thread only: idle streams
     uint64_t jobsDone = 0, posesDone = 0, evaluations = 0, hits = 0, misses = 0, evictions = 0;
@@ -262,8 +272,9 @@ bool att_server::evict_for(size_t need, const Job* j) {
     return true;
 }
 
-// Admits the job at the head of the queue if its footprint fits (engine
-// thread).  Returns false when it has to wait for memory.
+// Admits queued job j if its footprint fits (engine thread): removes it from
+// the queue and starts it, or fails it when it can never fit.  Returns false
+// when it has to wait for memory.
 bool att_server::admit(Job* j) {
     Receptor* r;
     Ligand* l;
@@ -280,7 +291,7 @@ bool att_server::admit(Job* j) {
         if (used + need > budget && !evict_for(need, j)) {
             if (!running.empty()) return false;
             // nothing will free memory: the job can never run
-            queue.pop_front();
+            queue.erase(std::find(queue.begin(), queue.end(), j));
             j->failed = true;
             j->error = "job needs " + std::to_string(need) + " bytes of device memory, budget " +
                        std::to_string(budget);
@@ -288,7 +299,8 @@ bool att_server::admit(Job* j) {
             cvDone.notify_all();
             return true;
         }
-        queue.pop_front();
+        queue.erase(std::find(queue.begin(), queue.end(), j));
+        j->admitSeq = ++admitted;
         used += j->bytes;  // receptor and ligand bytes are added on upload
         r->refs++;
         l->refs++;
@@ -419,15 +431,32 @@ void att_server::run() {
             }
         }
         bool progressed = false;
-        for (;;) {
+        // Admission in arrival order; past the oldest job waiting for memory
+        // (while it has been bypassed fewer than kMaxBypass times) later jobs
+        // are admitted when they fit.
+        for (size_t idx = 0;;) {
             Job* j;
             {
                 std::lock_guard<std::mutex> g(m);
-                if (queue.empty()) break;
-                j = queue.front();
+                if (idx >= queue.size()) break;
+                j = queue[idx];
             }
-            if (!admit(j)) break;
-            progressed = true;
+            if (admit(j)) {
+                progressed = true;
+                if (j == waiting) {
+                    waiting = nullptr;
+                    bypassed = 0;
+                } else if (idx > 0) {
+                    ++bypassed;
+                }
+                continue;  // the queue shrank at idx
+            }
+            if (idx == 0 && j != waiting) {
+                waiting = j;
+                bypassed = 0;
+            }
+            if (bypassed >= kMaxBypass) break;
+            ++idx;
         }
         for (size_t k = 0; k < running.size();) {
             Job* j = running[k];
@@ -644,6 +673,7 @@ att_status att_server_wait(att_server* s, uint64_t id, int timeout_ms, float* do
         info->grid_cached = j->gridCached ? 1 : 0;
         info->rounds = (unsigned)j->rounds;
         info->bytes = j->held;
+        info->admit_seq = j->admitSeq;
     }
     s->jobs.erase(it);
     return ATT_OK;

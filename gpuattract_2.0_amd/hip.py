@@ -122,6 +122,7 @@ def lib():
         L.att_ctx_create.argtypes = [C.c_int, C.POINTER(P)]
         L.att_set_pose_order.argtypes = [P, C.c_int]
         L.att_set_modes_waves.argtypes = [P, C.c_int]
+        L.att_debug_pair_counts.argtypes = [P, P, C.c_size_t]
         L.att_ctx_destroy.argtypes = [P]
         L.att_ctx_destroy.restype = None
         L.att_last_error.argtypes = [P]
@@ -282,6 +283,19 @@ class Context:
     def set_pose_order(self, enabled):
         """Spatial pose order of the single-pair adjoint (default on)."""
         self._chk(self.L.att_set_pose_order(self.h, 1 if enabled else 0))
+
+    def pair_counts(self, fn, n):
+        """Per-pose {listed neighbours walked, pairs inside the plateau} of the
+        scoring call fn() over n poses, from the production kernel: (n, 2) uint32."""
+        buf = self.dev_alloc(max(1, n) * 8)
+        self._chk(self.L.att_debug_pair_counts(self.h, buf.ptr, n))
+        try:
+            fn()
+            self.sync()
+            return self.to_host(buf, (n, 2), np.uint32)
+        finally:
+            self._chk(self.L.att_debug_pair_counts(self.h, None, 0))
+            buf.release()
 
     def set_modes_waves(self, waves):
         """Waves per pose of the flexible forward kernel (0: by ligand size)."""

@@ -23,7 +23,8 @@ _lib = None
 
 class JobInfo(C.Structure):
     _fields_ = [("queued_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
-                ("grid_cached", C.c_int), ("rounds", C.c_uint), ("bytes", C.c_size_t)]
+                ("grid_cached", C.c_int), ("rounds", C.c_uint), ("bytes", C.c_size_t),
+                ("admit_seq", C.c_uint64)]
 
 
 class Stats(C.Structure):

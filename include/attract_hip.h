@@ -107,6 +107,14 @@ att_status att_set_pose_order(att_ctx* ctx, int enabled);
  * the receptor mode gradients are summed per wave, then over the waves in
  * order, so their last bits do. */
 att_status att_set_modes_waves(att_ctx* ctx, int waves);
+/* Diagnostics (no counterpart in the reference): while d_counts is set, every
+ * att_score_rigid[_dev], att_score_modes[_dev] and att_score_adjoint[_dev]
+ * call writes, per pose, two uint32 {neighbour-list entries of the pose's
+ * ligand beads walked (the list lengths of their cells), pairs inside the
+ * plateau} to d_counts (device memory of `capacity` poses; a call with more
+ * poses fails).  NULL turns it off.  Integer sums: the same in every kernel
+ * and every order. */
+att_status att_debug_pair_counts(att_ctx* ctx, void* d_counts, size_t capacity);
 void att_ctx_destroy(att_ctx* ctx);
 const char* att_last_error(const att_ctx* ctx);
 

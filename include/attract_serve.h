@@ -44,6 +44,7 @@ typedef struct {
     int grid_cached;    /* the receptor grid was resident at admission */
     unsigned rounds;    /* scoring rounds issued */
     size_t bytes;       /* device memory the job held (state + buffers) */
+    uint64_t admit_seq; /* admission order: 1 for the server's first admitted job */
 } att_job_info;
 
 typedef struct {

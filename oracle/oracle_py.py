@@ -416,6 +416,26 @@ def beads(wl, dof):
     return out
 
 
+def pair_counts(wl, dofs=None):
+    """Per pose (n, 2) uint32: {neighbour-list entries of the cells of the
+    ligand beads inside the neighbour-list box, pairs inside the plateau}, from
+    the per-bead diagnostics of the NLPotForce restatement (model/
+    neighborlist.h:47-158: the list of the bead's cell is walked, pairs with
+    dr2 <= plateau^2 are evaluated)."""
+    d = wl.dofs if dofs is None else dofs
+    ge = wl.rec_nl.geom
+    cnt = wl.rec_nl.desc.reshape(-1, 2)[:, 0]
+    out = np.zeros((d.shape[0], 2), np.uint32)
+    for k in range(d.shape[0]):
+        b = beads(wl, d[k])
+        c = b["cell"][:, 6:9]
+        m = c[:, 0] >= 0
+        v = (c[m, 2].astype(np.int64) * ge.ny + c[m, 1]) * ge.nx + c[m, 0]
+        out[k, 0] = cnt[v].sum()
+        out[k, 1] = b["nl_pairs"].sum()
+    return out
+
+
 def rotmat(phi, ssi, rot):
     R = np.zeros(9, np.float32)
     oracle_lib().orc_euler2rotmat(phi, ssi, rot, _p(R))

@@ -158,18 +158,29 @@ def test_torch_harness_finite_differences(ctx, wl_adj_modes):
     assert np.allclose(g[:, :6], direct[:, 1:7], rtol=1e-6, atol=0)
     assert np.allclose(g[:, 6:], direct[:, 7:], rtol=1e-6, atol=0)
     assert table.grad is not None and np.abs(table.grad.numpy()).max() > 0
-    # central differences of the GPU energy in translation (h = 1e-3 A)
+    # central differences of the GPU's FP32 energy in translation and the
+    # Euler angles (h = 1e-3) and the receptor and ligand mode amplitudes
+    # (h = 1e-2 A: their gradients are smaller, so the energy's FP32 rounding
+    # needs the wider step); the energy is piecewise smooth (trilinear cells,
+    # plateau), so the median pins the derivative and a few poses may straddle
+    # a kink
     base = wl.dofs[:32].copy()
-    for col in (0, 1, 2):
+    cols = [(0, 1e-3, 2e-3), (1, 1e-3, 2e-3), (2, 1e-3, 2e-3), (3, 1e-3, 2e-3), (4, 1e-3, 2e-3),
+            (5, 1e-3, 2e-3)]
+    cols += [(6 + m, 1e-2, 2e-3) for m in range(wl.rec.M)]
+    cols += [(16 + m, 1e-2, 2e-3) for m in range(wl.lig.M)]
+    for col, h, tol in cols:
         dp, dm = base.copy(), base.copy()
-        dp[:, col] += 1e-3
-        dm[:, col] -= 1e-3
+        dp[:, col] += h
+        dm[:, col] -= h
         ep = ctx.adjoint(hs, dp)[:, 0].astype(np.float64)
         em = ctx.adjoint(hs, dm)[:, 0].astype(np.float64)
         fd = (ep - em) / (dp[:, col].astype(np.float64) - dm[:, col].astype(np.float64))
         an = direct[:, 1 + col] / wts.numpy()
         rel = np.abs(fd - an) / (np.abs(an) + np.abs(fd) + 1e-2)
-        assert np.median(rel) < 2e-3
+        print(f"DOF column {col}: median relative difference {np.median(rel):.1e}")
+        assert np.median(rel) < tol, col
+        assert np.mean(rel < 10 * tol) > 0.8, col
 
 
 def test_adjoint_vs_reference_forces(ctx, O, wl_adj_rigid):

@@ -139,6 +139,34 @@ def test_flexible_3000_800_all_poses_properties(ctx, flex):
           f"weight properties hold")
 
 
+def test_pair_counts_of_the_production_kernels(ctx, W, O, flex):
+    """Per-pose {neighbour-list entries walked, pairs inside the plateau}
+    written by k_rigid, k_modes and k_adjoint themselves (att_debug_pair_counts)
+    equal the oracle's integer counts: rigid 1000/300 (2,000 of its 10,000
+    poses) and flexible 3000/800 (500 poses); on all 100,000 flexible poses the
+    forward and adjoint kernels count the same pairs."""
+    rig = W.make_workload("rigid scoring 1000/300", 20261202, 1000, 300, 10000)
+    hr = ctx.upload_workload(rig)
+    m = 2000
+    want = O.pair_counts(rig, rig.dofs[:m])
+    for name, fn in (("k_rigid", lambda: ctx.score_rigid(hr, rig.dofs)),
+                     ("k_adjoint", lambda: ctx.adjoint(hr, rig.dofs))):
+        got = ctx.pair_counts(fn, rig.dofs.shape[0])
+        assert np.array_equal(got[:m], want), name
+        print(f"rigid 1000/300, {name}: {m} poses equal to the oracle; mean {got[:, 0].mean():.0f} "
+              f"list entries, {got[:, 1].mean():.0f} pairs inside the plateau per pose")
+    wl, hs = flex
+    k = 500
+    want = O.pair_counts(wl, wl.dofs[:k])
+    n = wl.dofs.shape[0]
+    fwd = ctx.pair_counts(lambda: ctx.score_modes(hs, wl.dofs), n)
+    adj = ctx.pair_counts(lambda: ctx.adjoint(hs, wl.dofs, pgrad=True, T=wl.params.T), n)
+    assert np.array_equal(fwd[:k], want) and np.array_equal(adj[:k], want)
+    assert np.array_equal(fwd, adj)
+    print(f"flexible 3000/800: k_modes and k_adjoint equal to the oracle on {k} poses and to each "
+          f"other on {n}; mean {fwd[:, 0].mean():.0f} list entries, {fwd[:, 1].mean():.0f} pairs")
+
+
 # ---------------------------------------------------------------- batched
 @pytest.fixture(scope="module")
 def batch(W, ctx):

@@ -105,13 +105,44 @@ def test_serve_budget_admission_and_eviction(ctx, S, pairs):
     with S.Server(0, budget) as srv:
         ids = [srv.add_workload(wl) for wl in wls]
         jobs = [(k % 2, srv.submit(ids[k % 2], wls[k % 2].dofs, evals[k % 2])) for k in range(4)]
-        for i, j in jobs:
-            assert same(srv.wait(j), ref[i])
+        res = [(i, srv.wait(j)) for i, j in jobs]
+        for i, r in res:
+            assert same(r, ref[i])
         st = srv.stats()
-        print(st)
+        print(st, [r[4]["admit_seq"] for _, r in res])
+        # one receptor fits at a time: the second pair's first job waits for
+        # memory, and the third job (first pair, receptor resident) may pass it
         assert st["peak_bytes"] <= budget
-        assert st["max_running"] == 1
-        assert st["evictions"] >= 3 and st["grid_misses"] == 4
+        assert st["evictions"] >= 1 and st["grid_misses"] >= 2
+        assert sorted(r[4]["admit_seq"] for _, r in res) == [1, 2, 3, 4]
+
+
+def test_serve_later_jobs_pass_a_job_waiting_for_memory(ctx, S, pairs, W):
+    """Budget for one large pair plus one small one: while the first large
+    job runs, a second large job (another receptor) waits for memory and a
+    small job submitted after it is admitted first; all three equal their
+    runs alone."""
+    big = [pairs[3], W.make_workload("serve e", 95, 1200, 300, 90)]
+    small = pairs[0]
+    evals = 60
+    peaks = []
+    for wl in big + [small]:
+        with S.Server(0) as srv:
+            srv.wait(srv.submit(srv.add_workload(wl), wl.dofs, evals))
+            peaks.append(srv.stats()["peak_bytes"])
+    budget = max(peaks[0], peaks[1]) + int(peaks[2] * 1.2)
+    assert budget < peaks[0] + peaks[1]
+    ref = [alone(ctx, wl, evals) for wl in big + [small]]
+    with S.Server(0, budget) as srv:
+        ids = [srv.add_workload(wl) for wl in big + [small]]
+        jobs = [srv.submit(i, wl.dofs, evals) for i, wl in zip(ids, big + [small])]
+        res = [srv.wait(j) for j in jobs]
+        seq = [r[4]["admit_seq"] for r in res]
+        print(f"budget {budget} B, peaks {peaks}, admission order {seq}")
+        for r, rr in zip(res, ref):
+            assert same(r, rr)
+        assert seq[0] == 1 and seq[2] == 2 and seq[1] == 3
+        assert srv.stats()["peak_bytes"] <= budget
 
 
 def test_serve_errors_and_timeouts(ctx, S, pairs):

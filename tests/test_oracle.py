@@ -57,6 +57,26 @@ def test_rigid_against_reference(W, O, name):
     assert (f["bead_npairs"] > 0).any()
 
 
+@pytest.mark.parametrize("name", RIGID)
+def test_pair_counts_against_reference(W, O, name):
+    """Per-pose pair counts {list entries walked, pairs inside the plateau}
+    from the reference's own per-bead values (its neighbour-list cells, the
+    list lengths of the generated grid, its per-bead pair counts), against the
+    oracle's."""
+    wl, f = _regenerate(W, name)
+    k = f["bead_cells"].shape[0]
+    got = O.pair_counts(wl, wl.dofs[:k])
+    ge = wl.rec_nl.geom
+    lengths = wl.rec_nl.desc.reshape(-1, 2)[:, 0]
+    for j in range(k):
+        c = f["bead_cells"][j][:, 6:9]
+        m = c[:, 0] >= 0
+        v = (c[m, 2].astype(np.int64) * ge.ny + c[m, 1]) * ge.nx + c[m, 0]
+        assert got[j, 0] == lengths[v].sum()
+        assert got[j, 1] == f["bead_npairs"][j].sum()
+    assert (got[:, 1] > 0).any() and (got[:, 0] >= got[:, 1]).all()
+
+
 @pytest.mark.parametrize("name", MODES)
 def test_modes_against_reference(W, O, name):
     wl, f = _regenerate(W, name)
