@@ -268,22 +268,24 @@ def main():
     # roofline numerator: the scoring path's algorithmic bytes per pose (bead
     # attributes, grid corners, neighbour-list descriptors and entries, mode
     # vectors, DOFs in and results out); the adjoint's own reads (it skips the
-    # receptor-on-ligand-grid lookups) and the parameter gradient's basis
-    # octets are reported beside it
+    # receptor-on-ligand-grid lookups) are reported beside it, and the
+    # parameter gradient's grid-cell records (20 bytes per ligand bead,
+    # written, sorted and reduced by the cell pass)
     bpp, bpp_sd = traffic.bytes_per_pose(wl)
     adj_bpp, _ = traffic.bytes_per_pose(wl, adjoint=True)
-    basis_bpp = (traffic.bytes_per_pose(wl, adjoint=True, pgrad=True)[0] - adj_bpp) if pgrad else 0.0
+    rec_bpp = 20.0 * wl.lig.n if pgrad else 0.0
     achieved = bpp * n / avg_launch
     prof = profiled_traffic(args.workload) if pgrad else None
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK, traffic=prof["bytes"] if prof else None,
                     traffic_source=prof["source"] if prof else None,
-                    kernel=f"k_adjoint<{5 if wl.modes else 0}> (+ parameter-gradient row reduction)",
+                    kernel=(f"k_adjoint<{5 if wl.modes else 0}>" +
+                            (" + the parameter gradient's cell pass and row reduction" if pgrad else "")),
                     bytes_per_pose=bpp, bytes_per_pose_sd=bpp_sd, poses_per_launch=n,
                     avg_launch_ms=avg_launch * 1e3,
                     adjoint_read_bytes_per_pose=adj_bpp,
-                    basis_octet_bytes_per_pose=basis_bpp,
-                    frac_with_adjoint_and_basis_reads=(adj_bpp + basis_bpp) * n / avg_launch / HBM_PEAK)
+                    param_grad_record_bytes_per_pose=rec_bpp,
+                    frac_adjoint_reads=adj_bpp * n / avg_launch / HBM_PEAK)
 
     # the same adjoint from host buffers in one call (DOFs over PCIe in, results
     # and parameter gradients out): the PCIe-inclusive rate, never `value`
