@@ -60,7 +60,8 @@ struct PGrad {
     unsigned T;
     float w;                      // upstream weight of the current pose
     unsigned* key;                // kPgRecords: per (slot, bead) grid-cell record
-    float4* pos;
+    float4* pos4;                 //   {x, y, z, w}, or
+    float3* pos3;                 //   {x, y, z} when the call has no weights
     unsigned cellsIn;             //   key = (cell, inner cells first) * T + type - 1
     unsigned invalid;             //   key of a bead without a grid term
 };
@@ -102,7 +103,10 @@ __device__ __forceinline__ float grid_adj_one(const DGrid& g, const DBasis& b, f
         if (type - 1u < pg.T) {
             const unsigned o = (iz * (g.ny - 1) + iy) * (g.nx - 1) + ix;
             pg.key[rec] = (cellBase + o) * pg.T + (type - 1);
-            pg.pos[rec] = make_float4(px, py, pz, pg.w);
+            if (pg.pos3)
+                pg.pos3[rec] = make_float3(px, py, pz);
+            else
+                pg.pos4[rec] = make_float4(px, py, pz, pg.w);
         } else {
             pg.key[rec] = pg.invalid;
         }
@@ -326,7 +330,7 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
     const float tx = d[0], ty = d[1], tz = d[2];
     const float wp = P.wts ? P.wts[pose] : 1.f;
     const PGrad pg{PG != kPgNone ? s_pg + wave * 3 * par.T * par.T : nullptr, tab, par.T, wp,
-                   P.recKey, P.recPos, P.recCellsIn, P.recInvalid};
+                   P.recKey, P.recPos4, P.recPos3, P.recCellsIn, P.recInvalid};
     float ampR[MA], ampL[MA];
 #pragma unroll
     for (int m = 0; m < MA; ++m) {

@@ -536,8 +536,11 @@ struct PairDesc {
     // Grid part of the parameter gradient as records (when set): slot i's bead
     // k writes recKey/recPos[i * lig.n + k]; keys (cell * T + type - 1, inner
     // cells first) below recInvalid, recInvalid for beads without a grid term.
+    // Positions as {x, y, z, w} (recPos4), or {x, y, z} when the call has no
+    // weights (recPos3; w = 1).
     unsigned* recKey;
-    float4* recPos;
+    float4* recPos4;
+    float3* recPos3;
     unsigned recCellsIn, recInvalid;
     // when set, per pose {listed neighbours of the beads walked, pairs inside
     // the plateau}

@@ -61,9 +61,10 @@ constexpr int kPgNone = 0, kPgFused = 1, kPgRecords = 2;
 // (keys, then positions) at the front, sort and reduction buffers after.
 struct PgCellBufs {
     unsigned* key;       // records as written by k_adjoint
-    float4* pos;
+    void* pos;           // float4 {x, y, z, w} per record, or float3 {x, y, z} without weights
     unsigned* keySorted;
-    float4* posSorted;
+    void* posSorted;
+    bool weights;
     unsigned* cellKey;   // runs of equal keys (per group of 64 sorted records), in order
     float* cellW;        // their summed corner weights, 8 per run
     unsigned* groupRuns; // runs per group of 64 records
@@ -71,8 +72,8 @@ struct PgCellBufs {
     void* temp;
     size_t tempBytes;
 };
-size_t pg_cells_bytes(size_t records, unsigned endBit);
-PgCellBufs pg_cells_layout(void* scratch, size_t records, unsigned endBit);
+size_t pg_cells_bytes(size_t records, unsigned endBit, bool weights);
+PgCellBufs pg_cells_layout(void* scratch, size_t records, unsigned endBit, bool weights);
 // Cell pass over the records of P's k_adjoint launch: sort by key, sum the
 // corner weights of each run of equal keys (fixed order), then `blocks`
 // one-wave workgroups contract every run with its cell's basis values and

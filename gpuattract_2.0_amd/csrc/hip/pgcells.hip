@@ -6,7 +6,8 @@
 // ipon(a,t) sum_cells sum_corners D_t(cell, corner) S_a(corner) / vol, where
 // D_t(cell, corner) sums w_pose * (corner weight) over the beads of type t in
 // that cell.  k_adjoint writes one record per bead (key = cell * T + t - 1,
-// {x, y, z, w}); here the records are sorted by key (stable radix sort), the
+// {x, y, z, w}, or {x, y, z} when the call has no pose weights); here the
+// records are sorted by key (stable radix sort), the
 // corner weights of each run of equal keys within a group of 64 sorted
 // records summed by a segmented scan across the wave, the run sums compacted
 // (integer scan of the per-group run counts), and each run sum contracted
@@ -37,6 +38,13 @@ struct CellGeom {
 
 __host__ __device__ inline CellGeom cell_geom(const DGrid& g) {
     return CellGeom{g.nx, g.ny, g.lox, g.loy, g.loz, g.dVox};
+}
+
+// A record's position and pose weight (1 for records without weights).
+__device__ inline float4 record_pos(const float4* pos, size_t i) { return pos[i]; }
+__device__ inline float4 record_pos(const float3* pos, size_t i) {
+    const float3 p = pos[i];
+    return make_float4(p.x, p.y, p.z, 1.f);
 }
 
 // Corner weights of a record {x, y, z, w} in cell `cell` (inner cells
@@ -71,10 +79,16 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 __host__ __device__ inline size_t groups_of(size_t records) { return (records + 63) / 64; }
 
-size_t temp_bytes(size_t records, unsigned endBit) {
+size_t temp_bytes(size_t records, unsigned endBit, bool weights) {
     size_t sortB = 0, scanB = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, sortB, (unsigned*)nullptr, (unsigned*)nullptr,
-                                    (float4*)nullptr, (float4*)nullptr, (unsigned)records, 0, endBit);
+    if (weights)
+        (void)rocprim::radix_sort_pairs(nullptr, sortB, (unsigned*)nullptr, (unsigned*)nullptr,
+                                        (float4*)nullptr, (float4*)nullptr, (unsigned)records, 0,
+                                        endBit);
+    else
+        (void)rocprim::radix_sort_pairs(nullptr, sortB, (unsigned*)nullptr, (unsigned*)nullptr,
+                                        (float3*)nullptr, (float3*)nullptr, (unsigned)records, 0,
+                                        endBit);
     (void)rocprim::exclusive_scan(nullptr, scanB, (const unsigned*)nullptr, (unsigned*)nullptr, 0u,
                                   groups_of(records), rocprim::plus<unsigned>());
     return align256(sortB > scanB ? sortB : scanB);
@@ -88,9 +102,9 @@ size_t temp_bytes(size_t records, unsigned endBit) {
 // between does), so the last lane of each run holds the run's sum.  COUNT
 // writes each group's number of runs (keys below `invalid`); otherwise the
 // runs go to cellKey/cellW from the group's offset, in lane order.
-template <bool COUNT>
+template <bool COUNT, class Pos>
 __global__ __launch_bounds__(256) void k_pg_runs(const unsigned* __restrict__ key,
-                                                 const float4* __restrict__ pos, size_t records,
+                                                 const Pos* __restrict__ pos, size_t records,
                                                  CellGeom in, CellGeom out, unsigned T,
                                                  unsigned cellsIn, unsigned invalid,
                                                  unsigned* __restrict__ groupRuns,
@@ -104,7 +118,7 @@ __global__ __launch_bounds__(256) void k_pg_runs(const unsigned* __restrict__ ke
     const unsigned k = i < records ? key[i] : 0xFFFFFFFFu;
     const bool live = k < invalid;
     CellW d{};
-    if (!COUNT && live) d = record_weights(in, out, cellsIn, k / T, pos[i]);
+    if (!COUNT && live) d = record_weights(in, out, cellsIn, k / T, record_pos(pos, i));
 #pragma unroll
     for (unsigned off = 1; off < 64; off <<= 1) {
         const unsigned ko = __shfl_up(k, off);
@@ -174,50 +188,55 @@ __global__ __launch_bounds__(64) void k_pgrad_cells(const unsigned* __restrict__
     for (unsigned k = lane; k < npg; k += 64) rows[(size_t)blockIdx.x * npg + k] = s_row[k];
 }
 
-size_t pg_cells_bytes(size_t records, unsigned endBit) {
-    const size_t R = records, G = groups_of(records);
-    return align256(4 * R) + align256(16 * R) + align256(4 * R) + align256(16 * R) +
+size_t pg_cells_bytes(size_t records, unsigned endBit, bool weights) {
+    const size_t R = records, G = groups_of(records), pb = weights ? 16 : 12;
+    return align256(4 * R) + align256(pb * R) + align256(4 * R) + align256(pb * R) +
            align256(4 * R) + align256(sizeof(CellW) * R) + 2 * align256(4 * G) +
-           temp_bytes(records, endBit);
+           temp_bytes(records, endBit, weights);
 }
 
-PgCellBufs pg_cells_layout(void* scratch, size_t records, unsigned endBit) {
-    const size_t R = records;
+PgCellBufs pg_cells_layout(void* scratch, size_t records, unsigned endBit, bool weights) {
+    const size_t R = records, pb = weights ? 16 : 12;
     char* p = static_cast<char*>(scratch);
     PgCellBufs B{};
+    B.weights = weights;
     B.key = reinterpret_cast<unsigned*>(p); p += align256(4 * R);
-    B.pos = reinterpret_cast<float4*>(p); p += align256(16 * R);
+    B.pos = p; p += align256(pb * R);
     B.keySorted = reinterpret_cast<unsigned*>(p); p += align256(4 * R);
-    B.posSorted = reinterpret_cast<float4*>(p); p += align256(16 * R);
+    B.posSorted = p; p += align256(pb * R);
     B.cellKey = reinterpret_cast<unsigned*>(p); p += align256(4 * R);
     B.cellW = reinterpret_cast<float*>(p); p += align256(sizeof(CellW) * R);
     B.groupRuns = reinterpret_cast<unsigned*>(p); p += align256(4 * groups_of(R));
     B.groupOffset = reinterpret_cast<unsigned*>(p); p += align256(4 * groups_of(R));
     B.temp = p;
-    B.tempBytes = temp_bytes(records, endBit);
+    B.tempBytes = temp_bytes(records, endBit, weights);
     return B;
 }
 
-hipError_t launch_pg_cells(const PairDesc& P, const PgCellBufs& B, size_t records, unsigned endBit,
-                           float* rows, unsigned blocks, hipStream_t s) {
+namespace {
+template <class Pos>
+hipError_t pg_cells_runs(const PairDesc& P, const PgCellBufs& B, size_t records, unsigned endBit,
+                         float* rows, unsigned blocks, hipStream_t s) {
     size_t tb = B.tempBytes;
-    hipError_t e = rocprim::radix_sort_pairs(B.temp, tb, B.key, B.keySorted, B.pos, B.posSorted,
+    Pos* pos = static_cast<Pos*>(B.pos);
+    Pos* posSorted = static_cast<Pos*>(B.posSorted);
+    hipError_t e = rocprim::radix_sort_pairs(B.temp, tb, B.key, B.keySorted, pos, posSorted,
                                              (unsigned)records, 0, endBit, s);
     if (e != hipSuccess) return e;
     const unsigned G = (unsigned)groups_of(records);
     const CellGeom in = cell_geom(P.inner), out = cell_geom(P.outer);
     const unsigned wgs = (G + 3) / 4;  // four groups (waves) per workgroup
-    hipLaunchKernelGGL(k_pg_runs<true>, dim3(wgs), dim3(256), 0, s, B.keySorted, B.posSorted, records,
-                       in, out, P.par.T, P.recCellsIn, P.recInvalid, B.groupRuns, nullptr, nullptr,
-                       nullptr);
+    hipLaunchKernelGGL((k_pg_runs<true, Pos>), dim3(wgs), dim3(256), 0, s, B.keySorted,
+                       (const Pos*)posSorted, records, in, out, P.par.T, P.recCellsIn, P.recInvalid,
+                       B.groupRuns, nullptr, nullptr, nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     tb = B.tempBytes;
     e = rocprim::exclusive_scan(B.temp, tb, (const unsigned*)B.groupRuns, B.groupOffset, 0u, G,
                                 rocprim::plus<unsigned>(), s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pg_runs<false>, dim3(wgs), dim3(256), 0, s, B.keySorted, B.posSorted,
-                       records, in, out, P.par.T, P.recCellsIn, P.recInvalid, B.groupRuns,
-                       B.groupOffset, B.cellKey, reinterpret_cast<CellW*>(B.cellW));
+    hipLaunchKernelGGL((k_pg_runs<false, Pos>), dim3(wgs), dim3(256), 0, s, B.keySorted,
+                       (const Pos*)posSorted, records, in, out, P.par.T, P.recCellsIn, P.recInvalid,
+                       B.groupRuns, B.groupOffset, B.cellKey, reinterpret_cast<CellW*>(B.cellW));
     if ((e = hipGetLastError()) != hipSuccess) return e;
     const size_t lds = sizeof(float) * 3 * (size_t)P.par.T * P.par.T;
     hipLaunchKernelGGL(k_pgrad_cells, dim3(blocks), dim3(64), lds, s, B.cellKey,
@@ -225,6 +244,13 @@ hipError_t launch_pg_cells(const PairDesc& P, const PgCellBufs& B, size_t record
                        P.inner, P.outer, P.bin, P.bout, P.par.tab, P.par.T, P.recCellsIn,
                        P.recInvalid, rows);
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_pg_cells(const PairDesc& P, const PgCellBufs& B, size_t records, unsigned endBit,
+                           float* rows, unsigned blocks, hipStream_t s) {
+    return B.weights ? pg_cells_runs<float4>(P, B, records, endBit, rows, blocks, s)
+                     : pg_cells_runs<float3>(P, B, records, endBit, rows, blocks, s);
 }
 
 }  // namespace att

@@ -293,8 +293,9 @@ namespace {
 // same bits.
 int pair_waves(unsigned ligN) { return ligN < 384 ? 1 : 2; }
 // Parameter-gradient records per call up to which the cell pass takes the
-// grid part (76 B of scratch per record: 10 GB at the cap), and the
-// workgroups of that pass (one parameter row each).
+// grid part (68 B of scratch per record without pose weights, 76 B with:
+// 9-10 GB at the cap), and the workgroups of that pass (one parameter row
+// each).
 constexpr size_t kPgRecordCap = (size_t)1 << 27;
 constexpr unsigned kPgCellBlocks = 2048;
 // Waves per pose of the flexible forward kernel (k_modes): with 384 ligand
@@ -780,7 +781,9 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
         return st;
     unsigned endBit = 0;
     while (endBit < 32 && (invalid >> endBit) != 0) ++endBit;
-    if (useRec && (st = grow(c, &c->d_rec, &c->d_rec_cap, pg_cells_bytes(records, endBit))) != ATT_OK)
+    const bool recW = d_w != nullptr;
+    if (useRec &&
+        (st = grow(c, &c->d_rec, &c->d_rec_cap, pg_cells_bytes(records, endBit, recW))) != ATT_OK)
         return st;
     if (n == 0) {
         if (pgrad) ATT_HIP(c, launch_fill_zero(d_pgrad, cols, s));
@@ -797,9 +800,12 @@ att_status att_score_adjoint_dev(att_ctx* c, att_handle hr, att_handle hl, att_h
     if ((st = pair_counts(c, n, &P.counts)) != ATT_OK) return st;
     PgCellBufs B{};
     if (useRec) {
-        B = pg_cells_layout(c->d_rec, records, endBit);
+        B = pg_cells_layout(c->d_rec, records, endBit, recW);
         P.recKey = B.key;
-        P.recPos = B.pos;
+        if (recW)
+            P.recPos4 = static_cast<float4*>(B.pos);
+        else
+            P.recPos3 = static_cast<float3*>(B.pos);
         P.recCellsIn = (unsigned)cellsIn;
         P.recInvalid = (unsigned)invalid;
     }

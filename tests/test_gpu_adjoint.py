@@ -264,3 +264,29 @@ def test_concurrent_streams_take_turns_on_scratch(W):
             assert np.array_equal(pg.view(np.uint32), alone[k][1].view(np.uint32))
     finally:
         ctx.close()
+
+
+def test_parameter_gradient_records_skip_beads_without_grid_term(ctx, W, O):
+    """The grid-cell records of beads without a grid term (type-0 beads; the
+    beads of poses moved outside both grids) are skipped: the parameter
+    gradient equals the FP64 oracle's to 1e-4 of its largest entry and the
+    per-bead basis path (the allocation-free entry point) to FP32 rounding;
+    the other outputs are the same bits on both paths."""
+    wl = W.make_workload("pg edges", 57, 500, 200, 400, 2, 2, with_basis=True, type0_frac=0.1)
+    wl.dofs[:50, :3] *= 4.0  # 50 poses far outside the grids
+    hs = _upload(ctx, wl)
+    T = wl.params.T
+    n = wl.dofs.shape[0]
+    g, pg = ctx.adjoint(hs, wl.dofs, pgrad=True, T=T)
+    _, pref = O.adjoint(wl, pgrad=True)
+    pg, pref = pg.reshape(-1), pref.reshape(-1)
+    assert np.abs(pg - pref).max() <= 1e-4 * np.abs(pref).max()
+    d_in = ctx.to_device(wl.dofs)
+    d_out = ctx.dev_alloc(n * 27 * 4)
+    d_pg = ctx.dev_alloc(3 * T * T * 4)
+    ctx.adjoint_dev_noalloc(hs, d_in.ptr, True, n, None, d_out.ptr, d_pg.ptr)
+    ctx.sync()
+    pb = ctx.to_host(d_pg, (3 * T * T,))
+    assert np.abs(pg - pb).max() <= 1e-5 * np.abs(pg).max()
+    assert np.array_equal(ctx.to_host(d_out, (n, 27)).view(np.uint32), g.view(np.uint32))
+    assert np.all(g[:50, 0] == 0.0)  # nothing in reach: no energy
