@@ -56,7 +56,7 @@ __device__ __forceinline__ void trilin_grad(const DGrid& g, float px, float py, 
 
 struct PGrad {
     float* s;                     // this wave's LDS slice, T*T*3, or nullptr when not requested
-    const att_pair_params* tab;   // the LDS copy of the pair table
+    PairTab tab;                  // the pair table
     unsigned T;
     float w;                      // upstream weight of the current pose
     unsigned* key;                // kPgRecords: per (slot, bead) grid-cell record
@@ -208,7 +208,7 @@ __device__ __forceinline__ void lj_dparam(float dr2, float dr2_inv, const att_pa
 //  the parameter derivatives into this wave's LDS slice.
 // Receptor beads are deformed from their padded mode row (one cache line).
 template <int MM, int PG>
-__device__ __forceinline__ void adj_walk(const PairDesc& P, const att_pair_params* tab,
+__device__ __forceinline__ void adj_walk(const PairDesc& P, const PairTab& tab,
                                          const PGrad& pg, float px, float py, float pz, float q,
                                          unsigned lt, uint2 d, const float* ampR, float* gR,
                                          float& eAcc, float3& gAcc, unsigned& inside) {
@@ -316,7 +316,7 @@ struct PoseLDS {
 // Every sum has a fixed order: results do not depend on timing.
 template <int MM, int WAVES, int PG>
 __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, PoseLDS<WAVES>& S,
-                                             const att_pair_params* tab, float* s_pg,
+                                             const PairTab& tab, float* s_pg,
                                              size_t recBase) {
     const DProt& lig = P.lig;
     const DPar& par = P.par;
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
     extern __shared__ att_pair_params s_dyn[];
     __shared__ PoseLDS<WAVES> S;
     char* dyn = reinterpret_cast<char*>(s_dyn);
-    const att_pair_params* tab = stage_table(P.par, s_dyn);
+    const PairTab tab = stage_table(P.par, s_dyn);
     // Parameter derivatives: one LDS slice per wave, so that every slice is
     // updated by one wave only, in program order; the slices are added in
     // wave order at the end.  Runs therefore give the same bits.
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(adjo
     __shared__ PoseLDS<WAVES> S;
     __shared__ unsigned s_tile;
     unsigned cur = 0xFFFFFFFFu;
-    const att_pair_params* tab = nullptr;
+    PairTab tab{nullptr, false};
     for (unsigned q = 0; q < kBatchQueues; ++q) {
         const unsigned x = (blockIdx.x + q) % kBatchQueues;
         const unsigned lo = queueStart[x], hi = queueStart[x + 1];

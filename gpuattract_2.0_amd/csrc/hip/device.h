@@ -292,6 +292,40 @@ __device__ __forceinline__ uint2 nl_cell(const DNL& nl, float px, float py, floa
     return nl.desc[nl.nx * (iz * nl.ny + iy) + ix];
 }
 
+// The pair table as the list walks read it: the workgroup's LDS copy
+// (stage_table) or the table in global memory.  Entries of the LDS copy are
+// read through an LDS-typed pointer, so they are LDS loads; through a generic
+// pointer they would be flat loads, which also wait for the vector-memory
+// loads of the receptor beads in flight.
+struct PairTab {
+    const att_pair_params* p;
+    bool lds;
+    __device__ __forceinline__ att_pair_params operator[](unsigned k) const {
+        if (lds) {
+            typedef const __attribute__((address_space(3))) float* LdsF;
+            typedef const __attribute__((address_space(3))) int* LdsI;
+            const LdsF f = (LdsF)(const float*)(p + k);
+            att_pair_params e;
+            e.rc = f[0];
+            e.ac = f[1];
+            e.ipon = ((LdsI)(const int*)(p + k))[2];
+            e.rmin2 = f[3];
+            e.emin = f[4];
+            return e;
+        }
+        return p[k];
+    }
+};
+
+// Pair table into LDS when it fits (kMaxTypesLDS), else read from global.
+__device__ __forceinline__ PairTab stage_table(const DPar& par, att_pair_params* s_tab) {
+    if (par.T <= kMaxTypesLDS) {
+        for (unsigned k = threadIdx.x; k < par.T * par.T; k += blockDim.x) s_tab[k] = par.tab[k];
+        return PairTab{s_tab, true};
+    }
+    return PairTab{par.tab, false};
+}
+
 // The pair loop of NLPotForce for one ligand bead (model/neighborlist.h:66-158
 // and the receptor reaction of model/neighborlist_modes.h:68-194) over the
 // list d = {numEl, idx}: every pair in list order, each term added to or
@@ -305,7 +339,7 @@ __device__ __forceinline__ uint2 nl_cell(const DNL& nl, float px, float py, floa
 // itself loads much, as a receptor deformed from its mode row).
 template <int NB = 4, class RecPos, class RecType, class RecCharge, class Reaction>
 __device__ __forceinline__ float4 nl_walk(const DNL& nl, const DPar& par,
-                                          const att_pair_params* tab, RecPos recPos,
+                                          const PairTab& tab, RecPos recPos,
                                           RecType recType, RecCharge recCharge,
                                           Reaction reaction, float px, float py, float pz,
                                           unsigned ligType, float ligCharge, uint2 d,
@@ -392,7 +426,7 @@ __device__ __forceinline__ float4 nl_walk(const DNL& nl, const DPar& par,
 // false when the bead has no neighbour-list term.
 template <class RecPos, class RecType, class RecCharge, class Reaction>
 __device__ __forceinline__ bool nl_bead(const DNL& nl, const DPar& par,
-                                        const att_pair_params* tab, RecPos recPos,
+                                        const PairTab& tab, RecPos recPos,
                                         RecType recType, RecCharge recCharge, Reaction reaction,
                                         float px, float py, float pz, unsigned ligType,
                                         float ligCharge, float4& acc, int* cell,
@@ -502,16 +536,6 @@ __device__ __forceinline__ float correct_mode(float d, float k) {
     const double dd = (double)d, d2 = dd * dd;
     const float cf = (float)((double)(4.f * k) * (d2 * d2));
     return d + cf;
-}
-
-// Pair table into LDS when it fits (kMaxTypesLDS), else read from global.
-__device__ __forceinline__ const att_pair_params* stage_table(const DPar& par,
-                                                              att_pair_params* s_tab) {
-    if (par.T <= kMaxTypesLDS) {
-        for (unsigned k = threadIdx.x; k < par.T * par.T; k += blockDim.x) s_tab[k] = par.tab[k];
-        return s_tab;
-    }
-    return par.tab;
 }
 
 // Byte offset of the adjoint's parameter-derivative slices behind the pair

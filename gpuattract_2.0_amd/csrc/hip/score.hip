@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
                                                const unsigned* __restrict__ order,
                                                uint2* __restrict__ counts) {
     extern __shared__ __align__(16) att_pair_params s_tab[];
-    const att_pair_params* tab = stage_table(par, s_tab);
+    const PairTab tab = stage_table(par, s_tab);
     __syncthreads();
     const unsigned lane = threadIdx.x & (kWave - 1);
     const unsigned wave = threadIdx.x / kWave;
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
     __shared__ float s_red[W][ATT_MODES_MAX];
     __shared__ float s_gl[ATT_MODES_MAX];
     __shared__ unsigned s_cnt[W][2];
-    const att_pair_params* tab = stage_table(par, s_tab);  // visible after phase B's first barrier
+    const PairTab tab = stage_table(par, s_tab);  // visible after phase B's first barrier
     if (blockIdx.x >= n) return;  // uniform per workgroup
     const unsigned pose = order ? order[xcd_slot(blockIdx.x, gridDim.x)] : blockIdx.x;
     const unsigned tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -387,8 +387,9 @@ __global__ void k_debug_rigid(DProt rec, DProt lig, DGrid inner, DGrid outer, DN
         float4 pp = pot_bead(inner, outer, p.x, p.y, p.z, lig.mtype[i], x.w, c);
         float4 a;
         unsigned np = 0;
-        if (nl_bead(nl, par, par.tab, RecPosG{rec.pos}, RecTypeG{rec.type}, RecChargeG{rec.pos},
-                    NoReaction{}, p.x, p.y, p.z, lig.type[i], x.w, a, c + 6, &np)) {
+        if (nl_bead(nl, par, PairTab{par.tab, false}, RecPosG{rec.pos}, RecTypeG{rec.type},
+                    RecChargeG{rec.pos}, NoReaction{}, p.x, p.y, p.z, lig.type[i], x.w, a, c + 6,
+                    &np)) {
             pp.x = pp.x + a.x; pp.y = pp.y + a.y; pp.z = pp.z + a.z; pp.w = pp.w + a.w;
         }
         xyz[i] = p.x; xyz[n + i] = p.y; xyz[2 * n + i] = p.z;
@@ -450,9 +451,9 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
         const float4 x = lig.pos[i];
         float4 a;
         unsigned np = 0;
-        if (nl_bead(nl, par, par.tab, recPos, RecTypeG{rec.type}, RecChargeG{rec.pos}, reaction,
-                    xyz[i], xyz[nL + i], xyz[2 * nL + i], lig.type[i], x.w, a, cell + 9 * i + 6,
-                    &np)) {
+        if (nl_bead(nl, par, PairTab{par.tab, false}, recPos, RecTypeG{rec.type},
+                    RecChargeG{rec.pos}, reaction, xyz[i], xyz[nL + i], xyz[2 * nL + i],
+                    lig.type[i], x.w, a, cell + 9 * i + 6, &np)) {
             pot[i] += a.x; pot[nL + i] += a.y; pot[2 * nL + i] += a.z; pot[3 * nL + i] += a.w;
         }
         npairs[i] = np;
