@@ -269,11 +269,12 @@ def main():
     # attributes, grid corners, neighbour-list descriptors and entries, mode
     # vectors, DOFs in and results out); the adjoint's own reads (it skips the
     # receptor-on-ligand-grid lookups) are reported beside it, and the
-    # parameter gradient's grid-cell records (20 bytes per ligand bead,
-    # written, sorted and reduced by the cell pass)
+    # parameter gradient's grid-cell records (16 bytes per ligand bead without
+    # pose weights: key and position, written, sorted and reduced by the cell
+    # pass)
     bpp, bpp_sd = traffic.bytes_per_pose(wl)
     adj_bpp, _ = traffic.bytes_per_pose(wl, adjoint=True)
-    rec_bpp = 20.0 * wl.lig.n if pgrad else 0.0
+    rec_bpp = 16.0 * wl.lig.n if pgrad else 0.0
     achieved = bpp * n / avg_launch
     prof = profiled_traffic(args.workload) if pgrad else None
     roofline = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
@@ -728,13 +729,13 @@ def batched_leg(args, np, hip, traffic, W, check=0):
     server = total / secs if secs else None
     if server:
         log(f"batched: drop-in Server path (2 workers, items of 100), pair after pair: {server:.4g} poses/s")
-    # the batched path builds no ligand grids: its bytes are the adjoint's
-    # reads; the forward formula (receptor beads on the ligand grids too) is
-    # reported beside it
-    roofline = dict(bound="hbm", achieved=algo / launch / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=algo / launch / HBM_PEAK, kernel="k_adjoint_batch",
-                    bytes_per_pose=algo / total, forward_formula_bytes_per_pose=algo_fwd / total,
-                    frac_forward_formula=algo_fwd / launch / HBM_PEAK)
+    # the scoring path's bytes per pose as for the single pair (the forward
+    # formula: receptor beads on the ligand grids too); the adjoint's own
+    # reads (it builds and reads no ligand grids) reported beside them
+    roofline = dict(bound="hbm", achieved=algo_fwd / launch / 1e9, peak=HBM_PEAK / 1e9,
+                    unit="GB/s", frac=algo_fwd / launch / HBM_PEAK, kernel="k_adjoint_batch",
+                    bytes_per_pose=algo_fwd / total, adjoint_read_bytes_per_pose=algo / total,
+                    frac_adjoint_reads=algo / launch / HBM_PEAK)
     prof = profiled_traffic("batched")
     if prof:
         roofline.update(traffic=prof["bytes"], traffic_source=prof["source"])
