@@ -119,8 +119,6 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
     }
 }
 
-constexpr int kModesVals = 13 + 2 * ATT_MODES_MAX;
-
 // The deformed receptor position of bead j from its padded mode row (one or
 // two 16-byte loads), in the order of deform(): bit-identical coordinates.
 template <int MM>
