@@ -294,6 +294,28 @@ att_status att_body_desc(const att_body* b, att_protein_desc* out) {
 
 void att_body_free(att_body* b) { delete b; }
 
+att_status att_read_alphabet(const char* path, unsigned* out, unsigned capacity, unsigned* n) {
+    if (!path || !n) return err(ATT_ERR_INVALID, "null argument");
+    std::ifstream in(path);
+    if (!in.is_open()) return err(ATT_ERR_INVALID, std::string("Error: Failed to open file ") + path);
+    std::vector<unsigned> types;
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ls(line);
+        int key = 0;
+        if (!(ls >> key) || key < 0)
+            return err(ATT_ERR_INVALID, std::string(path) + ": line " +
+                                            std::to_string(types.size() + 1) +
+                                            " is not a non-negative bead type");
+        types.push_back((unsigned)key);
+    }
+    *n = (unsigned)types.size();
+    if (!out) return ATT_OK;
+    if (types.size() > capacity) return err(ATT_ERR_INVALID, "alphabet capacity too small");
+    std::copy(types.begin(), types.end(), out);
+    return ATT_OK;
+}
+
 att_status att_read_params(const char* path, att_pair_params* table, unsigned capacity,
                            unsigned* numTypes, int* potShape, float* swiOn, float* swiOff) {
     if (!path || !numTypes) return err(ATT_ERR_INVALID, "null argument");

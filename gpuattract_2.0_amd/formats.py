@@ -38,6 +38,7 @@ def lib():
         L.att_read_params.argtypes = [C.c_char_p, P, C.c_uint, C.POINTER(C.c_uint),
                                       C.POINTER(C.c_int), C.POINTER(C.c_float),
                                       C.POINTER(C.c_float)]
+        L.att_read_alphabet.argtypes = [C.c_char_p, P, C.c_uint, C.POINTER(C.c_uint)]
         L.att_dof_read.argtypes = [C.c_char_p, C.POINTER(P)]
         L.att_dof_get_info.argtypes = [P, C.POINTER(DofInfo)]
         L.att_dof_ligand.argtypes = [P, P, P, C.c_int, C.c_uint, C.c_uint, P]
@@ -119,6 +120,15 @@ def read_params(path):
     _chk(lib().att_read_params(os.fsencode(path), hip._p(tab), len(tab), C.byref(T), C.byref(shape),
                                C.byref(on), C.byref(off)))
     return T.value, shape.value, on.value, off.value, tab
+
+
+def read_alphabet(path):
+    """The bead types of a grid alphabet file, in file order (uint32)."""
+    n = C.c_uint(0)
+    _chk(lib().att_read_alphabet(os.fsencode(path), None, 0, C.byref(n)))
+    out = np.zeros(n.value, np.uint32)
+    _chk(lib().att_read_alphabet(os.fsencode(path), hip._p(out), len(out), C.byref(n)))
+    return out
 
 
 class DofFile:

@@ -54,6 +54,12 @@ void att_body_free(att_body* b);
 att_status att_read_params(const char* path, att_pair_params* table, unsigned capacity,
                            unsigned* numTypes, int* potShape, float* swiOn, float* swiOff);
 
+/* --- grid alphabet files: readGridAlphabetFromFile (readFile.cpp:901-926) ----
+ * One non-negative integer (a bead type) per line, in the order that
+ * att_body_map_types takes.  With out NULL only *n is set (size query);
+ * capacity = entries of `out`. */
+att_status att_read_alphabet(const char* path, unsigned* out, unsigned capacity, unsigned* n);
+
 /* --- DOF files: readDOFHeader (readFile.cpp:766-871) + readDOF (:579-651) ---
  * then DOFConverter[_Modes] (model/DOFConverter.h) and transformDOF_glob2rec
  * (model/DOFTransform.cpp:36-84) for a receptor-ligand pair. */

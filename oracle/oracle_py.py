@@ -182,6 +182,11 @@ def ref_lib():
                               C.POINTER(C.c_size_t), P]
         L.ref_format_results.restype = C.c_size_t
         L.ref_format_results.argtypes = [P, C.c_size_t, C.c_int, C.c_uint, C.c_uint, P, C.c_size_t]
+        L.ref_alphabet.restype = C.c_int
+        L.ref_alphabet.argtypes = [C.c_char_p, P, C.c_uint, C.POINTER(C.c_uint)]
+        L.ref_score_files.restype = C.c_int
+        L.ref_score_files.argtypes = [C.c_char_p] * 7 + [C.c_double, P, C.c_size_t,
+                                                         C.POINTER(C.c_size_t)]
         L.ref_gridfile_load.restype = C.c_int
         L.ref_gridfile_load.argtypes = [C.c_char_p]
         L.ref_gridfile_info.restype = None
@@ -575,6 +580,29 @@ def ref_format_results(res, modes=False, Mr=0, Ml=0):
     buf = C.create_string_buffer(max(1, ln))
     L.ref_format_results(_p(r), r.shape[0], 1 if modes else 0, Mr, Ml, buf, ln)
     return buf.raw[:ln].decode()
+
+
+def ref_alphabet(path, cap=1000):
+    """A grid alphabet file through the reference's reader."""
+    L = ref_lib()
+    out = np.zeros(cap, np.uint32)
+    n = C.c_uint(0)
+    if L.ref_alphabet(os.fsencode(path), _p(out), cap, C.byref(n)) != 0:
+        raise RuntimeError("alphabet larger than the buffer")
+    return out[:n.value].copy()
+
+
+def ref_score_files(dof, rec, lig, grid, par, alphabet, dielec="variable", epsilon=15.0,
+                    cap=100000):
+    """The reference's scoring app chain on files (Configurator_6D<float> +
+    CPUEnergyService6D behind as::Server): (n, 7) results."""
+    L = ref_lib()
+    res = np.zeros((cap, 7), np.float32)
+    n = C.c_size_t(0)
+    args = [os.fsencode(x) for x in (dof, rec, lig, grid, par, alphabet, dielec)]
+    if L.ref_score_files(*args, epsilon, _p(res), cap, C.byref(n)) != 0:
+        raise RuntimeError("more poses than the result buffer")
+    return res[:n.value].copy()
 
 
 def ref_read_gridfile(path):

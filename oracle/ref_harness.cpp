@@ -21,6 +21,11 @@
 //  grid files     : fileIO/readFile.cpp:260-422 (createGridFromGridFile over
 //                   attract::Grid::read_std) on a file written by this
 //                   project's att_gridfile_write
+//  scoring from files: the reference's own chain of the scoring app
+//                   (service/Configurator_6D.tpp:31-141: readers, type
+//                   mapping from the grid alphabet, pivots, DOF transform,
+//                   grid displacement, CPUEnergyService6D behind as::Server),
+//                   as test/app/ScoringTest.cpp drives it
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -39,6 +44,8 @@
 #include <sstream>
 
 #include "CPUEnergyService6D.h"
+#include "CmdArgs.h"
+#include "Configurator_6D.h"
 #include "DOFConverter.h"
 #include "DOFTransform.h"
 #include "DataManager.h"
@@ -55,6 +62,8 @@
 #include "matrixFunctions.h"
 #include "neighborlist_modes.h"
 #include "readFile.h"
+#include "Request.h"
+#include "Server.h"
 #include "reduction_modes.h"
 #include "transform_modes.h"
 
@@ -605,6 +614,54 @@ int ref_dof(const char* path, int modes, unsigned Mr, unsigned Ml, const float* 
                 d[16 + k] = k < Ml ? q.modesLig[k] : 0.f;
             }
         }
+    }
+    return 0;
+}
+
+// A grid alphabet file through readGridAlphabetFromFile (readFile.cpp:901-926).
+int ref_alphabet(const char* path, unsigned* out, unsigned cap, unsigned* n) {
+    const std::vector<unsigned> v = readGridAlphabetFromFile(path);
+    *n = (unsigned)v.size();
+    if (v.size() > cap) return -1;
+    std::copy(v.begin(), v.end(), out);
+    return 0;
+}
+
+// Rigid scoring from files, as the scoring app and test/app/ScoringTest.cpp:
+// Configurator_6D<float>::init on the given files (one CPU worker,
+// CPUEnergyService6D), one Request over every DOF of the file through
+// as::Server, results {E, pos, ang} per pose.  dielec "variable" or
+// "constant", epsilon as the command line's.
+int ref_score_files(const char* dof, const char* rec, const char* lig, const char* grid,
+                    const char* par, const char* alphabet, const char* dielec, double epsilon,
+                    float* res, size_t cap, size_t* n) {
+    CmdArgs args;
+    args.app = AppType::SCORE;
+    args.dofName = dof;
+    args.recName = rec;
+    args.ligName = lig;
+    args.gridRecName = grid;
+    args.paramsName = par;
+    args.alphabetRecName = alphabet;
+    args.numModes = 0;
+    args.numCPUs = 1;
+    args.chunkSize = 1000;
+    args.precision = "single";
+    args.dielec = dielec;
+    args.epsilon = epsilon;
+    Configurator_6D<float> conf;
+    conf.init(args);
+    auto& dofs = conf.dofs();
+    *n = dofs.size();
+    if (dofs.size() > cap) return -1;
+    Request<DOF_6D<float>, Common> request(dofs.data(), dofs.size(), conf.common());
+    conf.server()->submit(request);
+    std::vector<Result_6D<float>> out(dofs.size());
+    conf.server()->wait(request, out.data());
+    for (size_t i = 0; i < out.size(); ++i) {
+        res[7 * i] = out[i].E;
+        res[7 * i + 1] = out[i].pos.x; res[7 * i + 2] = out[i].pos.y; res[7 * i + 3] = out[i].pos.z;
+        res[7 * i + 4] = out[i].ang.x; res[7 * i + 5] = out[i].ang.y; res[7 * i + 6] = out[i].ang.z;
     }
     return 0;
 }

@@ -187,6 +187,19 @@ def test_result_text_matches_reference(F, R, modes):
     assert F.format_results(res, modes, Mr, Ml) == R.ref_format_results(res, modes, Mr, Ml)
 
 
+def test_alphabet_matches_reference_reader(F, R, tmp_path):
+    path = tmp_path / "rec.alphabet"
+    path.write_text(" 3\n1\n 12 \n2\n99\n")
+    ours = F.read_alphabet(str(path))
+    assert list(ours) == [3, 1, 12, 2, 99]
+    assert np.array_equal(ours, R.ref_alphabet(str(path)))
+    from gpuattract_amd import hip
+    for bad in ("1\n-2\n", "1\nx\n"):
+        path.write_text(bad)
+        with pytest.raises(hip.AttError, match="not a non-negative bead type"):
+            F.read_alphabet(str(path))
+
+
 def test_type_mapping(W, F, tmp_path):
     body = W.make_body(75, 1, 60, W.scaled_radius(60, W.LIG_REF), 6, 0.2, 0.1)
     path = str(tmp_path / "map.pdb")
