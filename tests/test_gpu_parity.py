@@ -1,8 +1,14 @@
 """HIP path vs the CPU oracle on the same seeded inputs (needs a GPU).
 
 Bit-exact: rotation matrices, transformed coordinates, per-bead forces and
-energies, grid cell indices, neighbour-list cells and pair counts.
-Tolerance (tests/parity.py): per-pose sums within 1e-5 of the condition scale.
+energies, grid cell indices, neighbour-list cells and pair counts, and the
+per-pose energy, force, torque and ligand mode gradients of the forward
+kernels (the oracle folds each bead's pairs in list order and the beads in
+bead order, as the kernels and the reference do); the adjoint against the
+oracle's kernel-order restatement.  The forward's receptor mode gradients
+(the reaction projected per pair) within 1e-5 of their norm.  The
+tolerance of tests/parity.py (within 1e-5 of the condition scale) is
+asserted beside, with the FP64 comparison printed.
 """
 import numpy as np
 import pytest
@@ -135,8 +141,27 @@ def _rigid_parity(ctx, wl, O, nthreads=8):
     e32 = cond_err(g[ok], o32[ok], sc[ok])
     print(summarize("rigid gpu vs oracle fp32", g[ok], o32[ok], sc[ok]))
     assert e32.max() <= TOL, e32.max(axis=0)
+    # every pose, clamped forces included: the oracle sums in the kernels'
+    # (the reference's) order, so the bits are equal
+    assert np.array_equal(g.view(np.uint32), o32.view(np.uint32))
     _fp64_statement("rigid", g[ok], o32[ok], o64[ok], sc[ok])
     return g, o32
+
+
+def _assert_modes_bits(g, o32, Mr):
+    """Energy, force, torque and ligand mode gradients equal the oracle's
+    bits; the receptor mode gradients (the reaction projected per pair, not
+    per receptor bead as the reference sums it) within 1e-5 of their norm."""
+    keep = list(range(7)) + list(range(17, 27))
+    assert np.array_equal(g[:, keep].view(np.uint32), o32[:, keep].view(np.uint32))
+    if Mr:
+        r = g[:, 7:7 + Mr].astype(np.float64)
+        q = o32[:, 7:7 + Mr].astype(np.float64)
+        fin = np.all(np.isfinite(q), axis=1)
+        assert np.array_equal(fin, np.all(np.isfinite(r), axis=1))
+        nrm = np.linalg.norm(q[fin], axis=1)
+        err = np.linalg.norm(r[fin] - q[fin], axis=1)
+        assert np.all(err <= 1e-5 * nrm + 1e-30), (err / np.maximum(nrm, 1e-30)).max()
 
 
 def test_score_rigid_small(ctx, wl_rigid, O):
@@ -159,6 +184,7 @@ def test_score_modes_small(ctx, wl_modes, O):
     e = cond_err(g[ok][:, cols], o32[ok][:, cols], sc[ok][:, cols])
     print(summarize("modes gpu vs oracle fp32", g[ok][:, cols], o32[ok][:, cols], sc[ok][:, cols]))
     assert e.max() <= TOL, e.max(axis=0)
+    _assert_modes_bits(g, o32, wl_modes.rec.M)
     _fp64_statement("modes", g[ok][:, cols], o32[ok][:, cols], o64[ok][:, cols], sc[ok][:, cols])
     # unused mode slots are exactly zero
     assert np.all(g[:, 7 + wl_modes.rec.M:17] == 0) and np.all(g[:, 17 + wl_modes.lig.M:] == 0)
@@ -229,15 +255,14 @@ def test_edge_cases(ctx, W, O):
         o64, sc = O.score_rigid(wl, f64=True)
         ok = forcelim_safe(o64)
         assert cond_err(g[ok], o32[ok], sc[ok]).max() <= TOL
+        assert np.array_equal(g.view(np.uint32), o32.view(np.uint32))
     wl = W.make_workload("one bead", 22, 100, 1, 32)
     hs = ctx.upload_workload(wl)
     g = ctx.score_rigid(hs, wl.dofs)
     o64, sc = O.score_rigid(wl, f64=True)
-    # one bead: the only summation-order freedom left is its neighbour-list
-    # pairs, which the scoring kernel sums as a tree (one pair per lane) and
-    # the oracle in list order; the per-bead values are bit-exact through the
-    # diagnostics path (_check_beads)
-    assert cond_err(g, O.score_rigid(wl), sc).max() <= TOL
+    # one bead: its neighbour-list pairs folded in list order, as the oracle
+    # (and the reference) fold them
+    assert np.array_equal(g.view(np.uint32), O.score_rigid(wl).view(np.uint32))
     assert cond_err(g, o64, sc).max() <= 1e-3
     # poses far outside every grid score exactly zero
     far = wl.dofs.copy()
@@ -267,40 +292,53 @@ def test_edge_cases(ctx, W, O):
 def test_many_types_table_in_global_memory(ctx, W, O):
     """40 bead types: the pair table (1600 entries) exceeds what the kernels
     stage in LDS (32x32), so they read it from global memory and their LDS
-    slot arrays start at offset 0; forward and adjoint still match the oracle."""
+    slot arrays start at offset 0; forward and adjoint still equal the
+    oracle's bits (the forward's receptor mode gradients to 1e-5 normwise)."""
     wl = W.make_workload("many types", 31, 300, 90, 64, num_types=40)
     hs = ctx.upload_workload(wl)
     g = ctx.score_rigid(hs, wl.dofs)
     o64, sc = O.score_rigid(wl, f64=True)
     ok = forcelim_safe(o64)
     assert cond_err(g[ok], O.score_rigid(wl)[ok], sc[ok]).max() <= TOL
+    assert np.array_equal(g.view(np.uint32), O.score_rigid(wl).view(np.uint32))
     a = ctx.adjoint(hs, wl.dofs)
     r32 = O.adjoint(wl, f32=True)
     _, sca = O.adjoint(wl, scale=True)
     cols = [c for c in range(7) if sca[:, c].max() > 0]
     assert cond_err(a[:, cols], r32[:, cols], sca[:, cols]).max() <= TOL
+    k = O.adjoint_kernel_order(wl)[:, :7].astype(np.float32)
+    assert np.array_equal(a.view(np.uint32), k.view(np.uint32))
     wm = W.make_workload("many types modes", 32, 300, 90, 32, 2, 3, num_types=40)
     hm = ctx.upload_workload(wm)
     gm = ctx.score_modes(hm, wm.dofs)
+    om = O.score_modes(wm)
     _, scm = O.score_modes(wm, f64=True)
     colsm = scm.max(axis=0) > 0
-    assert cond_err(gm[:, colsm], O.score_modes(wm)[:, colsm], scm[:, colsm]).max() <= TOL
+    assert cond_err(gm[:, colsm], om[:, colsm], scm[:, colsm]).max() <= TOL
+    _assert_modes_bits(gm, om, wm.rec.M)
+    am = ctx.adjoint(hm, wm.dofs)
+    km = O.adjoint_kernel_order(wm)[:, :27].astype(np.float32)
+    assert np.array_equal(am.view(np.uint32), km.view(np.uint32))
 
 
 @pytest.mark.parametrize("n_lig", [63, 64, 65, 129])
 def test_bead_chunk_boundaries(ctx, W, O, n_lig):
-    """Ligands around the 64-bead chunks of the rigid kernel's pair-parallel
-    phase and the 128-bead rounds of the modes kernel: forward results match
-    the oracle."""
+    """Ligands around the 64-bead chunks of the scoring kernels' lane-per-bead
+    phases: forward results equal the oracle's bits (the receptor mode
+    gradients to 1e-5 normwise)."""
     wl = W.make_workload(f"chunks {n_lig}", 40 + n_lig, 250, n_lig, 48)
     hs = ctx.upload_workload(wl)
     g = ctx.score_rigid(hs, wl.dofs)
     o64, sc = O.score_rigid(wl, f64=True)
     ok = forcelim_safe(o64)
-    assert cond_err(g[ok], O.score_rigid(wl)[ok], sc[ok]).max() <= TOL
+    o32 = O.score_rigid(wl)
+    assert cond_err(g[ok], o32[ok], sc[ok]).max() <= TOL
+    assert np.array_equal(g.view(np.uint32), o32.view(np.uint32))
     wm = W.make_workload(f"chunks modes {n_lig}", 50 + n_lig, 250, n_lig, 24, 2, 2)
     hm = ctx.upload_workload(wm)
     gm = ctx.score_modes(hm, wm.dofs)
+    om = O.score_modes(wm)
     _, scm = O.score_modes(wm, f64=True)
     cols = scm.max(axis=0) > 0
-    assert cond_err(gm[:, cols], O.score_modes(wm)[:, cols], scm[:, cols]).max() <= TOL
+    assert cond_err(gm[:, cols], om[:, cols], scm[:, cols]).max() <= TOL
+    _assert_modes_bits(gm, om, wm.rec.M)
