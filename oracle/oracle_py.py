@@ -187,6 +187,10 @@ def ref_lib():
         L.ref_score_files.restype = C.c_int
         L.ref_score_files.argtypes = [C.c_char_p] * 7 + [C.c_double, P, C.c_size_t,
                                                          C.POINTER(C.c_size_t)]
+        L.ref_score_files_modes.restype = C.c_int
+        L.ref_score_files_modes.argtypes = [C.c_char_p] * 10 + [C.c_int, C.c_char_p, C.c_double,
+                                                                  P, C.c_size_t,
+                                                                  C.POINTER(C.c_size_t)]
         L.ref_gridfile_load.restype = C.c_int
         L.ref_gridfile_load.argtypes = [C.c_char_p]
         L.ref_gridfile_info.restype = None
@@ -601,6 +605,22 @@ def ref_score_files(dof, rec, lig, grid, par, alphabet, dielec="variable", epsil
     n = C.c_size_t(0)
     args = [os.fsencode(x) for x in (dof, rec, lig, grid, par, alphabet, dielec)]
     if L.ref_score_files(*args, epsilon, _p(res), cap, C.byref(n)) != 0:
+        raise RuntimeError("more poses than the result buffer")
+    return res[:n.value].copy()
+
+
+def ref_score_files_modes(dof, rec, lig, grid_rec, grid_lig, par, alphabet_rec, alphabet_lig,
+                          modes_rec, modes_lig, num_modes, dielec="variable", epsilon=15.0,
+                          cap=100000):
+    """The reference's flexible scoring chain on files (Configurator_6D_Modes
+    + CPUEnergyService6DModes behind as::Server): (n, 27) results."""
+    L = ref_lib()
+    res = np.zeros((cap, 27), np.float32)
+    n = C.c_size_t(0)
+    paths = [os.fsencode(x) for x in (dof, rec, lig, grid_rec, grid_lig, par, alphabet_rec,
+                                      alphabet_lig, modes_rec, modes_lig)]
+    if L.ref_score_files_modes(*paths, num_modes, os.fsencode(dielec), epsilon, _p(res), cap,
+                               C.byref(n)) != 0:
         raise RuntimeError("more poses than the result buffer")
     return res[:n.value].copy()
 

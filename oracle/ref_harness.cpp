@@ -25,7 +25,9 @@
 //                   (service/Configurator_6D.tpp:31-141: readers, type
 //                   mapping from the grid alphabet, pivots, DOF transform,
 //                   grid displacement, CPUEnergyService6D behind as::Server),
-//                   as test/app/ScoringTest.cpp drives it
+//                   as test/app/ScoringTest.cpp drives it; for flexible
+//                   scoring the input steps of Configurator_6D_Modes.tpp and
+//                   the modes model chain
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -350,6 +352,11 @@ struct ModesSetup {
           gridRec(make_igrid(*recInner), make_igrid(*recOuter), make_nlgrid(*recNL)),
           gridLig(make_igrid(*ligInner), make_igrid(*ligOuter), make_nlgrid(*ligNL)),
           pt(make_table(*tab)), sp(make_sim(*tab)) {}
+    ModesSetup(std::shared_ptr<Protein<float>> rec, std::shared_ptr<Protein<float>> lig,
+               const GridUnion<float>& gRec, const GridUnion<float>& gLig,
+               std::shared_ptr<ParamTable<float>> table, std::shared_ptr<SimParam<float>> sim)
+        : pr(rec), pl(lig), gridRec(gRec.inner, gRec.outer, gRec.NL),
+          gridLig(gLig.inner, gLig.outer, gLig.NL), pt(table), sp(sim) {}
 };
 
 static void score_modes_range(const ModesSetup& S, const float* dof, size_t lo, size_t hi, float* res,
@@ -663,6 +670,89 @@ int ref_score_files(const char* dof, const char* rec, const char* lig, const cha
         res[7 * i + 1] = out[i].pos.x; res[7 * i + 2] = out[i].pos.y; res[7 * i + 3] = out[i].pos.z;
         res[7 * i + 4] = out[i].ang.x; res[7 * i + 5] = out[i].ang.y; res[7 * i + 6] = out[i].ang.z;
     }
+    return 0;
+}
+
+// Flexible scoring from files: the input steps of
+// Configurator_6D_Modes<float>::init (service/Configurator_6D_Modes.tpp:
+// readers, the ligand's types mapped with the receptor grid's alphabet,
+// DOFConverter_Modes, pivots, transformDOF_glob2rec, the receptor grid
+// displaced by the receptor pivot, numModes modes read for both bodies, the
+// receptor's types mapped with the ligand grid's alphabet, the ligand grid
+// displaced by the ligand pivot), each through the reference's own function,
+// then the modes model chain above (the order of
+// CPUEnergyService6DModes.tpp:121-327).  The service object itself is not
+// used: it sizes its worker buffers crosswise and fails its WorkerBuffer
+// index assertion.  One change of order: Common_Modes::numModesRec/Lig are
+// set before DOFConverter_Modes, which reads them (the configurator sets them
+// only after converting, so the app converts with their previous values, 0 in
+// a fresh process, and reads every amplitude as 0).  Results {E, pos, ang,
+// modesRec[10], modesLig[10]}.
+int ref_score_files_modes(const char* dof, const char* rec, const char* lig, const char* gridRec,
+                          const char* gridLig, const char* par, const char* alphabetRec,
+                          const char* alphabetLig, const char* recModes, const char* ligModes,
+                          int numModes, const char* dielec, double epsilon, float* res, size_t cap,
+                          size_t* n) {
+    auto receptor = createProteinFromPDB<float>(rec);
+    auto ligand = createProteinFromPDB<float>(lig);
+    auto paramTable = createParamTableFromFile<float>(par);
+    auto gRec = createGridFromGridFile<float>(gridRec);
+    auto simParam = std::make_shared<SimParam<float>>();
+    simParam->dielec = std::string(dielec) == "variable" ? Dielec::variable : Dielec::constant;
+    simParam->epsilon = static_cast<float>(epsilon);
+    simParam->ffelec = static_cast<float>(FELEC / epsilon);
+    TypeMap mapRec = createTypeMapFromVector(readGridAlphabetFromFile(alphabetRec));
+    ligand->setNumMappedTypes(1);
+    ligand->getOrCreateMappedPtr();
+    applyDefaultMapping(ligand->numAtoms(), ligand->type(), ligand->type());
+    applyMapping(mapRec, ligand->numAtoms(), ligand->type(), ligand->mappedType());
+    DOFHeader<float> h = readDOFHeader<float>(dof);
+    std::vector<std::vector<DOF>> raw = readDOF(dof);
+    Common_Modes::numModesLig = numModes;
+    Common_Modes::numModesRec = numModes;
+    std::vector<std::vector<DOF_6D_Modes<float>>> mol = DOFConverter_Modes<float>(raw);
+    if (mol.size() != 2) return -2;
+    if (h.auto_pivot) {
+        receptor->auto_pivotize();
+        ligand->auto_pivotize();
+        h.pivots.push_back(receptor->pivot());
+        h.pivots.push_back(ligand->pivot());
+    } else {
+        if (h.pivots.size() != 2) return -2;
+        receptor->pivotize(h.pivots[0]);
+        ligand->pivotize(h.pivots[1]);
+    }
+    transformDOF_glob2rec(mol[0], mol[1], h.pivots[0], h.pivots[1], h.centered_receptor,
+                          h.centered_ligands);
+    gRec->translate(-make_real3(receptor->pivot().x, receptor->pivot().y, receptor->pivot().z));
+    receptor->setNumModes(numModes);
+    ligand->setNumModes(numModes);
+    readHMMode<float>(receptor, recModes);
+    readHMMode<float>(ligand, ligModes);
+    TypeMap mapLig = createTypeMapFromVector(readGridAlphabetFromFile(alphabetLig));
+    receptor->setNumMappedTypes(1);
+    receptor->getOrCreateMappedPtr();
+    applyDefaultMapping(receptor->numAtoms(), receptor->type(), receptor->type());
+    applyMapping(mapLig, receptor->numAtoms(), receptor->type(), receptor->mappedType());
+    auto gLig = createGridFromGridFile<float>(gridLig);
+    gLig->translate(-make_real3(ligand->pivot().x, ligand->pivot().y, ligand->pivot().z));
+    // the configurator's DOF buffer: pose, then the modes of each body
+    const size_t np = mol[1].size();
+    *n = np;
+    if (np > cap) return -1;
+    std::vector<float> d(26 * np, 0.f);
+    for (size_t i = 0; i < np; ++i) {
+        const DOF_6D_Modes<float>& q = mol[1][i];
+        float* o = &d[26 * i];
+        o[0] = q._6D.pos.x; o[1] = q._6D.pos.y; o[2] = q._6D.pos.z;
+        o[3] = q._6D.ang.x; o[4] = q._6D.ang.y; o[5] = q._6D.ang.z;
+        for (int k = 0; k < numModes; ++k) {
+            o[6 + k] = q.modesRec[k];
+            o[16 + k] = q.modesLig[k];
+        }
+    }
+    ModesSetup S(receptor, ligand, *gRec, *gLig, paramTable, simParam);
+    score_modes_range(S, d.data(), 0, np, res, nullptr, nullptr);
     return 0;
 }
 
