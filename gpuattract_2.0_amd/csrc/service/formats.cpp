@@ -466,6 +466,21 @@ att_status att_dof_get_info(const att_dof_file* f, att_dof_info* info) {
     return ATT_OK;
 }
 
+att_status att_dof_values(const att_dof_file* f, unsigned molecule, size_t pose, double* values,
+                          unsigned* count) {
+    if (!f || !values || !count) return err(ATT_ERR_INVALID, "null argument");
+    if (molecule >= f->mol.size() || pose >= f->mol[molecule].size())
+        return err(ATT_ERR_INVALID, "no such molecule or pose in the DOF file");
+    const DofRec& d = f->mol[molecule][pose];
+    for (int k = 0; k < 3; ++k) {
+        values[k] = d.ang[k];
+        values[3 + k] = d.pos[k];
+    }
+    for (unsigned k = 0; k < d.nExtra; ++k) values[6 + k] = d.extra[k];
+    *count = 6 + d.nExtra;
+    return ATT_OK;
+}
+
 att_status att_dof_ligand(const att_dof_file* f, const float* pivotRec, const float* pivotLig,
                           int modes, unsigned numModesRec, unsigned numModesLig, void* out) {
     if (!f || !out) return err(ATT_ERR_INVALID, "null argument");

@@ -41,6 +41,7 @@ def lib():
         L.att_read_alphabet.argtypes = [C.c_char_p, P, C.c_uint, C.POINTER(C.c_uint)]
         L.att_dof_read.argtypes = [C.c_char_p, C.POINTER(P)]
         L.att_dof_get_info.argtypes = [P, C.POINTER(DofInfo)]
+        L.att_dof_values.argtypes = [P, C.c_uint, C.c_size_t, P, C.POINTER(C.c_uint)]
         L.att_dof_ligand.argtypes = [P, P, P, C.c_int, C.c_uint, C.c_uint, P]
         L.att_dof_free.argtypes = [P]
         L.att_dof_free.restype = None
@@ -147,6 +148,14 @@ class DofFile:
         i = DofInfo()
         _chk(lib().att_dof_get_info(self.h, C.byref(i)))
         return i
+
+    def values(self, molecule, pose):
+        """The values of one line as read (float64): phi, ssi, rot, x, y, z,
+        then the mode amplitudes."""
+        v = np.zeros(16, np.float64)
+        n = C.c_uint(0)
+        _chk(lib().att_dof_values(self.h, molecule, pose, hip._p(v), C.byref(n)))
+        return v[:n.value].copy()
 
     def ligand(self, pivot_rec=None, pivot_lig=None, modes=False, num_modes_rec=0,
                num_modes_lig=0):

@@ -74,6 +74,12 @@ typedef struct {
 } att_dof_info;
 att_status att_dof_read(const char* path, att_dof_file** out);
 att_status att_dof_get_info(const att_dof_file* f, att_dof_info* info);
+/* One line of the file as readDOF reads it (readFile.cpp:579-651): the
+ * values of molecule `molecule` in pose block `pose` into values[0..15]
+ * (phi, ssi, rot, x, y, z, then the mode amplitudes), their number (6 plus
+ * the amplitudes) into *count. */
+att_status att_dof_values(const att_dof_file* f, unsigned molecule, size_t pose, double* values,
+                          unsigned* count);
 /* Ligand DOFs in the receptor frame (receptor pose folded into the ligand's,
  * pivots and centering applied) in float, as att_dof6 (modes = 0) or
  * att_dof6m records (modes = 1: receptor mode amplitudes from the receptor's
