@@ -33,7 +33,6 @@ struct DGrid {
 struct DNL {
     const uint2* desc;
     const uint32_t* list;
-    const float* ratio;
     unsigned nx, ny, nz;
     float lox, loy, loz, hix, hiy, hiz;
     float dVox_inv, P2, P2_inv;
@@ -233,6 +232,39 @@ __device__ __forceinline__ float4 pot_bead(const DGrid& inner, const DGrid& oute
     return pot;
 }
 
+// pot_bead without branches around the lookup: the grid (inner, else outer)
+// is selected per lane and its octet always read, the result replaced by zero
+// where the bead is in neither grid or has mapped type 0.  The same values as
+// pot_bead; a loop over beads can have several lookups in flight.  The read
+// stays inside the grid for any coordinate (grid_index clamps to the last
+// voxel, NaN included) and type (mtype < ngrids as uploaded).
+__device__ __forceinline__ float4 pot_bead_flat(const DGrid& inner, const DGrid& outer, float px,
+                                                float py, float pz, unsigned mtype, float charge) {
+    const bool in = !grid_out(inner, px, py, pz);
+    const bool valid = mtype != 0 && (in || !grid_out(outer, px, py, pz));
+    DGrid g;
+    g.oct = in ? inner.oct : outer.oct;
+    g.nx = in ? inner.nx : outer.nx;
+    g.ny = in ? inner.ny : outer.ny;
+    g.nz = in ? inner.nz : outer.nz;
+    g.lox = in ? inner.lox : outer.lox;
+    g.loy = in ? inner.loy : outer.loy;
+    g.loz = in ? inner.loz : outer.loz;
+    g.dVox = in ? inner.dVox : outer.dVox;
+    g.dVox_inv = in ? inner.dVox_inv : outer.dVox_inv;
+    g.vol_inv = in ? inner.vol_inv : outer.vol_inv;
+    const unsigned ix = grid_index(px, g.lox, g.dVox_inv, g.nx);
+    const unsigned iy = grid_index(py, g.loy, g.dVox_inv, g.ny);
+    const unsigned iz = grid_index(pz, g.loz, g.dVox_inv, g.nz);
+    float4 pot = trilin(g, px, py, pz, ix, iy, iz, mtype);
+    if (fabsf(charge) > 0.001f) {  // interp's electrostatic term
+        float4 el = trilin(g, px, py, pz, ix, iy, iz, 0);
+        el.x = el.x * charge; el.y = el.y * charge; el.z = el.z * charge; el.w = el.w * charge;
+        pot.x = pot.x + el.x; pot.y = pot.y + el.y; pot.z = pot.z + el.z; pot.w = pot.w + el.w;
+    }
+    return valid ? pot : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // LJPotForce (model/forcefield.h:28-83) with swi = 1: 6.0*vlj and the force
 // factor are evaluated in double, as the reference's FP32 build does.
 __device__ __forceinline__ void lj(float dr2, float dr2_inv, float dx, float dy, float dz,
@@ -326,6 +358,16 @@ __device__ __forceinline__ PairTab stage_table(const DPar& par, att_pair_params*
     return PairTab{par.tab, false};
 }
 
+// Entry n of the plateau-ratio table, sqrt(((n + 0.5) / 10000) / P2) in double
+// rounded to float (model/NLGrid.tpp:44-51, read by getRatio at n =
+// int(dr2 * 10000), model/NLGrid.h:141-143).  Evaluated per pair rather than
+// loaded: a division, a division and a square root, each correctly rounded
+// in double as on the host, so the value is the table's to the bit (checked
+// over every entry), and the walk waits on no load after the distance.
+__device__ __forceinline__ float plateau_ratio(unsigned n, float P2) {
+    return (float)__builtin_sqrt((((double)n + 0.5) / 10000.0) / (double)P2);
+}
+
 // The pair loop of NLPotForce for one ligand bead (model/neighborlist.h:66-158
 // and the receptor reaction of model/neighborlist_modes.h:68-194) over the
 // list d = {numEl, idx}: every pair in list order, each term added to or
@@ -379,7 +421,9 @@ __device__ __forceinline__ float4 nl_walk(const DNL& nl, const DPar& par,
         }
 #pragma unroll
         for (int k = 0; k < NB; ++k)
-            ratios[k] = (j0 + k < d.x && dr2s[k] <= nl.P2) ? nl.ratio[(int)(dr2s[k] * 10000.f)] : 0.f;
+            ratios[k] = (j0 + k < d.x && dr2s[k] <= nl.P2)
+                            ? plateau_ratio((unsigned)(int)(dr2s[k] * 10000.f), nl.P2)
+                            : 0.f;
 #pragma unroll
         for (int k = 0; k < NB; ++k) {
             if (j0 + k >= d.x) break;
@@ -674,23 +718,27 @@ __device__ __forceinline__ float reduce_beads_ref(float red, unsigned lane, cons
     const unsigned fc = lane == 0 ? 3u : lane < 4u ? lane - 1u : (lane - 4u) / 3u;
     const unsigned xc = lane < 4u ? 3u : (lane - 4u) % 3u;  // 3: no coordinate factor
     const float* ff = reinterpret_cast<const float*>(f);
-    const float4* px = pos + first;
+    // The coordinates are the same for every lane and read-only: read through
+    // the constant address space, they are scalar loads (one per wave, from
+    // the scalar cache) rather than vector loads of one address.
+    typedef const __attribute__((address_space(4))) float* ConstF;
+    const ConstF pc = (ConstF)(const float*)(pos + first);
+    auto px = [pc](unsigned i) { return make_float4(pc[4 * i], pc[4 * i + 1], pc[4 * i + 2], pc[4 * i + 3]); };
     auto coord = [xc](const float4& x) { return xc == 0u ? x.x : xc == 1u ? x.y : xc == 2u ? x.z : 1.f; };
     unsigned k = 0;
-    // eight beads' loads are issued together (the coordinates are the same for
-    // every lane: scalar loads); the sum stays sequential
+    // eight beads' loads are issued together; the sum stays sequential
     for (; k + 8 <= m; k += 8) {
         float4 xs[8];
         float fv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            xs[u] = px[k + u];
+            xs[u] = px(k + u);
             fv[u] = ff[4 * (k + u) + fc];
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) red += coord(xs[u]) * fv[u];  // 1 * f = f exactly
     }
-    for (; k < m; ++k) red += coord(px[k]) * ff[4 * k + fc];
+    for (; k < m; ++k) red += coord(px(k)) * ff[4 * k + fc];
     return red;
 }
 

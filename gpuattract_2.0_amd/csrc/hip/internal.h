@@ -37,6 +37,7 @@ hipError_t launch_debug_modes(const DProt& rec, const DProt& lig, const DGrid& r
 
 hipError_t launch_debug_rotmats(const float* ang, unsigned n, float* R, hipStream_t s);
 hipError_t launch_debug_sincosf(const float* x, unsigned n, float* sn, float* cs, hipStream_t s);
+hipError_t launch_debug_plateau_ratio(float P2, unsigned n, float* ratio, hipStream_t s);
 
 hipError_t launch_copy(void* dst, const void* src, size_t n16, hipStream_t s);
 

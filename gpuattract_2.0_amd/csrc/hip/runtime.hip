@@ -6,8 +6,9 @@
 //            modes [c][m][n] (coalesced over beads) and [n][m][c] (one run per
 //            bead, for neighbour-list gathers), modeForce[M]
 //   grid     inner/outer float4 [grid][z][y][x] exactly as the reference's
-//            IntrplGrid, neighbour-list {numEl, idx}[z][y][x] + u32 list, and
-//            the plateau-ratio table of model/NLGrid.tpp:44-51
+//            IntrplGrid, neighbour-list {numEl, idx}[z][y][x] + u32 list (the
+//            plateau-ratio table of model/NLGrid.tpp:44-51 is evaluated per
+//            pair by the kernels, not stored)
 //   params   att_pair_params[T*T] + shape/dielectric/ffelec
 #include <hip/hip_runtime.h>
 
@@ -225,25 +226,16 @@ att_status expand_octets(att_ctx* c, Obj* o, const att_grid_desc& g, const float
     return ATT_OK;
 }
 
-// Neighbour-list constants and the plateau-ratio table (model/NLGrid.tpp:
-// 44-51) of a grid whose desc/list are already on the device.
+// Neighbour-list constants of a grid whose desc/list are already on the
+// device.
 att_status attach_nl(att_ctx* c, GridObj* o, unsigned nx, unsigned ny, unsigned nz, float ox,
                      float oy, float oz, float spacing, float plateau, const uint2* ddesc,
                      const uint32_t* dlist) {
+    (void)c;  // nothing to allocate
     const float P2 = plateau * plateau;
-    const int size = (int)(10000 * P2);
-    std::vector<float> ratio((size_t)size + 1);
-    for (int k = 0; k <= size; ++k) {
-        const double d2 = ((k + 0.5) / 10000);
-        ratio[k] = (float)std::sqrt(d2 / (double)P2);
-    }
-    float* dratio;
-    att_status st;
-    if ((st = dev_copy(c, o, ratio.data(), ratio.size(), &dratio)) != ATT_OK) return st;
     DNL& d = o->nl;
     d.desc = ddesc;
     d.list = dlist;
-    d.ratio = dratio;
     d.nx = nx; d.ny = ny; d.nz = nz;
     const float dv = spacing;
     d.lox = ox; d.loy = oy; d.loz = oz;
@@ -1036,6 +1028,20 @@ att_status att_debug_sincosf(att_ctx* c, const float* x, size_t n, float* sn, fl
     ATT_HIP(c, hipStreamSynchronize(c->stream));
     ATT_HIP(c, hipMemcpy(sn, ds, sizeof(float) * n, hipMemcpyDeviceToHost));
     ATT_HIP(c, hipMemcpy(cs, dc, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return ATT_OK;
+}
+
+att_status att_debug_plateau_ratio(att_ctx* c, float P2, size_t n, float* ratio) {
+    if (!c || (n && !ratio)) return fail(c, ATT_ERR_INVALID, "null argument");
+    if (n == 0) return ATT_OK;
+    if (n > (1u << 30)) return fail(c, ATT_ERR_UNSUPPORTED, "too many entries");
+    ATT_HIP(c, hipSetDevice(c->device));
+    Tmp t;
+    float* dr;
+    ATT_HIP(c, t.alloc(&dr, n));
+    ATT_HIP(c, launch_debug_plateau_ratio(P2, (unsigned)n, dr, c->stream));
+    ATT_HIP(c, hipStreamSynchronize(c->stream));
+    ATT_HIP(c, hipMemcpy(ratio, dr, sizeof(float) * n, hipMemcpyDeviceToHost));
     return ATT_OK;
 }
 

@@ -243,7 +243,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
 #pragma unroll
     for (int m = 0; m < MA; ++m) mR[m] = 0.f;
 
-    // Phase A: receptor beads.
+    // Phase A: receptor beads, two lookups in flight per lane (pot_bead_flat).
+#pragma unroll 2
     for (unsigned j = tid; j < rec.n; j += nthr) {
         const float4 x = rec.pos[j];
         BeadModes<MM> md;
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
         const V3 df = deform(md, rec.M, x.x, x.y, x.z, ampR);
         V3 p = matvec(Rinv, df.x, df.y, df.z);
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
-        const float4 pot = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
+        const float4 pot = pot_bead_flat(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w);
         const V3 f = matvec(R, pot.x, pot.y, pot.z);  // rotate_forces (transform_modes.h:141-161)
         load_modes(rec, j, md);  // reloaded (cache hit) rather than held across the lookup
         mode_dots(md, rec.M, f.x, f.y, f.z, mR);
@@ -474,6 +475,11 @@ __global__ void k_debug_sincosf(const float* x, unsigned n, float* sn, float* cs
     if (i < n) sincosf_libm(x[i], sn + i, cs + i);
 }
 
+__global__ void k_debug_plateau_ratio(float P2, unsigned n, float* ratio) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ratio[i] = plateau_ratio(i, P2);
+}
+
 // Expands a [grid][z][y][x] float4 grid into voxel octets (see DGrid).
 __global__ __launch_bounds__(256) void k_octets(const float4* __restrict__ g,
                                                 float4* __restrict__ oct, unsigned nx,
@@ -596,6 +602,12 @@ hipError_t launch_debug_modes(const DProt& rec, const DProt& lig, const DGrid& r
 hipError_t launch_debug_sincosf(const float* x, unsigned n, float* sn, float* cs, hipStream_t s) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_debug_sincosf, dim3((n + 255) / 256), dim3(256), 0, s, x, n, sn, cs);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_plateau_ratio(float P2, unsigned n, float* ratio, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_debug_plateau_ratio, dim3((n + 255) / 256), dim3(256), 0, s, P2, n, ratio);
     return hipGetLastError();
 }
 

@@ -141,6 +141,7 @@ def lib():
         L.att_debug_beads_modes.argtypes = [P, H, H, H, H, H, P, P, P, P, P, P, P]
         L.att_debug_rotmats.argtypes = [P, P, C.c_size_t, P]
         L.att_debug_sincosf.argtypes = [P, P, C.c_size_t, P, P]
+        L.att_debug_plateau_ratio.argtypes = [P, C.c_float, C.c_size_t, P]
         L.att_param_grad_rmin_emin.argtypes = [P, C.c_uint, C.c_int, P, P]
         L.att_score_adjoint_dev_noalloc.argtypes = [P, H, H, H, H, P, C.c_int, C.c_size_t, P, P, P, P]
         L.att_sync.argtypes = [P, P]
@@ -537,3 +538,10 @@ class Context:
         c = np.zeros_like(a)
         self._chk(self.L.att_debug_sincosf(self.h, _p(a), a.size, _p(s), _p(c)))
         return s, c
+
+    def plateau_ratio(self, P2, n):
+        """Entries 0..n-1 of the plateau-ratio table of plateau^2 P2 as the
+        kernels evaluate them per pair."""
+        r = np.zeros(n, np.float32)
+        self._chk(self.L.att_debug_plateau_ratio(self.h, float(P2), n, _p(r)))
+        return r

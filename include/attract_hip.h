@@ -272,6 +272,12 @@ att_status att_debug_rotmats(att_ctx* ctx, const float* angles /* 3n */, size_t 
  * checking them against the host library. */
 att_status att_debug_sincosf(att_ctx* ctx, const float* x, size_t n, float* s, float* c);
 
+/* Entries 0..n-1 of the neighbour-list plateau-ratio table of plateau^2 = P2
+ * as the kernels evaluate them per pair (model/NLGrid.tpp:44-51 builds the
+ * table, model/NLGrid.h:141-143 reads entry int(dr2 * 10000)), for checking
+ * them against the table. */
+att_status att_debug_plateau_ratio(att_ctx* ctx, float P2, size_t n, float* ratio);
+
 att_status att_sync(att_ctx* ctx, void* stream);
 
 /* A plain float4 copy of `bytes` (multiple of 16) from d_src to d_dst on

@@ -72,6 +72,22 @@ def test_device_sincosf_is_the_c_library(ctx):
     assert np.all(np.isnan(s)) and np.all(np.isnan(c))
 
 
+@pytest.mark.parametrize("plateau", [10.0, 4.25, 6.0, 7.5, 8.8, 10.8, 12.0, 15.3])
+def test_plateau_ratio_is_the_table(ctx, O, plateau):
+    """The plateau ratio the list walks evaluate per pair equals every entry of
+    the table the reference builds (model/NLGrid.tpp:44-51): the oracle's
+    table (pinned to the reference's golden values at plateau 10) and, where
+    oracle/_ref is built, the reference NLGrid's own table."""
+    lut = O.ratio_lut(plateau)
+    P2 = np.float32(plateau) * np.float32(plateau)
+    assert len(lut) == int(np.float32(10000) * P2) + 1
+    dev = ctx.plateau_ratio(P2, len(lut))
+    assert np.array_equal(dev.view(np.uint32), lut.view(np.uint32))
+    if O.ref_available():
+        ref = O.ref_ratio_lut(plateau)
+        assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
+
+
 def test_rotmats_bit_exact(ctx, O):
     """Rotation matrices as the kernels build them equal the oracle's (the
     reference's formula on the C library's sin/cos) and the reference's own
