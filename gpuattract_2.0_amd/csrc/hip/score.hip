@@ -121,12 +121,15 @@ __global__ __launch_bounds__(256) void k_rigid(DProt rec, DProt lig, DGrid inner
 
 // The deformed receptor position of bead j from its padded mode row (one or
 // two 16-byte loads), in the order of deform(): bit-identical coordinates.
+// The row is also left in `rowOut` (4 * ceil(3M / 4) floats) for the pair's
+// reaction, which then does not load it again.
 template <int MM>
 struct RecPosDeform {
     const float4* pos;
     const float4* modesT;
     unsigned rowT, M;
     const float* amp;
+    float* rowOut;
     __device__ float3 operator()(unsigned j) const {
         const float4 x = pos[j];
         float yx = x.x, yy = x.y, yz = x.z;
@@ -140,6 +143,8 @@ struct RecPosDeform {
                 const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
                 mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
             }
+#pragma unroll
+            for (unsigned i = 0; i < 4 * Q; ++i) rowOut[i] = mt[i];
 #pragma unroll
             for (unsigned m = 0; m < (unsigned)MA; ++m)
                 if (m < M) {
@@ -186,7 +191,8 @@ __device__ __forceinline__ float reduce_modes_ref(float red, const float* RT, co
     return red;
 }
 
-// One workgroup of W waves per pose (W = 1 below 384 ligand beads, else 2).
+// One workgroup of W waves per pose (W = 1 below 384 ligand beads, else 4;
+// att_set_modes_waves overrides it).
 // Phase A: every receptor bead deformed and moved into the ligand frame
 // (invertDOF, rotate_translate_deform), scored on the ligand grid, its force
 // rotated back (rotate_forces) and projected on the receptor modes.  Phase B:
@@ -262,15 +268,14 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
     const float4* __restrict__ rpos = rec.pos;
     const float4* __restrict__ rmodesT = rec.modesT;
     const unsigned Mr = rec.M, rowT = rec.rowT;
-    auto reaction = [&](unsigned j, float fx, float fy, float fz) {
-        const float4* row = rmodesT + (size_t)j * rowT;
-        constexpr unsigned Q = (3 * MA + 3) / 4;
-        float mt[4 * Q];
-#pragma unroll
-        for (unsigned q = 0; q < Q; ++q) {
-            const float4 v = q < rowT ? row[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-            mt[4 * q] = v.x; mt[4 * q + 1] = v.y; mt[4 * q + 2] = v.z; mt[4 * q + 3] = v.w;
-        }
+    // The walk takes one list entry at a time, so the reaction of a pair
+    // follows the deformation of the same receptor bead: its mode row is the
+    // one RecPosDeform left in rowBuf.
+    constexpr int kWalkBatch = 1;
+    static_assert(kWalkBatch == 1, "the reaction reads the row of the last deformed bead");
+    float rowBuf[4 * ((3 * MA + 3) / 4)];
+    auto reaction = [&](unsigned, float fx, float fy, float fz) {
+        const float* mt = rowBuf;
 #pragma unroll
         for (unsigned m = 0; m < (unsigned)MM; ++m)
             if (m < Mr) mR[m] -= fx * mt[3 * m] + fy * mt[3 * m + 1] + fz * mt[3 * m + 2];
@@ -308,9 +313,10 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
                 float4 x;
                 const V3 p = place(sc + k, x);
                 unsigned np = 0;
-                const float4 a = nl_walk<1>(nl, par, tab, RecPosDeform<MM>{rpos, rmodesT, rowT, Mr, ampR},
-                                         RecTypeG{rec.type}, RecChargeG{rpos}, reaction, p.x, p.y,
-                                         p.z, lig.type[sc + k], x.w, s_n[k], &np);
+                const float4 a = nl_walk<kWalkBatch>(
+                    nl, par, tab, RecPosDeform<MM>{rpos, rmodesT, rowT, Mr, ampR, rowBuf},
+                    RecTypeG{rec.type}, RecChargeG{rpos}, reaction, p.x, p.y, p.z,
+                    lig.type[sc + k], x.w, s_n[k], &np);
                 listed += s_n[k].x;
                 inside += np;
                 float4 f = s_f[k];
