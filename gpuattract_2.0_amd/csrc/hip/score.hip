@@ -260,7 +260,6 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(MM >= 10
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
         const float4 pot = pot_bead_flat(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w);
         const V3 f = matvec(R, pot.x, pot.y, pot.z);  // rotate_forces (transform_modes.h:141-161)
-        load_modes(rec, j, md);  // reloaded (cache hit) rather than held across the lookup
         mode_dots(md, rec.M, f.x, f.y, f.z, mR);
     }
 
