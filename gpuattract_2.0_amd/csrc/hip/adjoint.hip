@@ -413,8 +413,10 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
         }
         if (wave == 0) {
             for (unsigned k = lane; k < m; k += kWave) {
-                float4 x;
-                const V3 xt = deformed(sc + k, x);
+                const float4 x = lig.pos[sc + k];
+                BeadModes<MM> md;  // loaded once for the deformation and the mode gradient
+                load_modes(lig, sc + k, md);
+                const V3 xt = deform(md, Ml, x.x, x.y, x.z, ampL);
                 const float4 G = S.g[k];
                 acc[0] += G.x; acc[1] += G.y; acc[2] += G.z;
                 acc[3] += G.x * xt.x; acc[4] += G.x * xt.y; acc[5] += G.x * xt.z;
@@ -425,8 +427,6 @@ __device__ __forceinline__ void adjoint_pose(const PairDesc& P, unsigned pose, P
                     const float gx = R[0] * G.x + R[3] * G.y + R[6] * G.z;
                     const float gy = R[1] * G.x + R[4] * G.y + R[7] * G.z;
                     const float gz = R[2] * G.x + R[5] * G.y + R[8] * G.z;
-                    BeadModes<MM> md;
-                    load_modes(lig, sc + k, md);
                     mode_dots(md, Ml, -gx, -gy, -gz, gL);  // mode_dots subtracts
                 }
             }
