@@ -237,7 +237,8 @@ __device__ __forceinline__ float4 pot_bead(const DGrid& inner, const DGrid& oute
 // where the bead is in neither grid or has mapped type 0.  The same values as
 // pot_bead; a loop over beads can have several lookups in flight.  The read
 // stays inside the grid for any coordinate (grid_index clamps to the last
-// voxel, NaN included) and type (mtype < ngrids as uploaded).
+// voxel, NaN included) and type (the scoring entry points refuse bodies with
+// a mapped type >= the grid's number of grids).
 __device__ __forceinline__ float4 pot_bead_flat(const DGrid& inner, const DGrid& outer, float px,
                                                 float py, float pz, unsigned mtype, float charge) {
     const bool in = !grid_out(inner, px, py, pz);
