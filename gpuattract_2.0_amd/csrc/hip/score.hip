@@ -428,7 +428,8 @@ __global__ void k_debug_modes(DProt rec, DProt lig, DGrid rInner, DGrid rOuter, 
         V3 p = matvec(Rinv, df.x, df.y, df.z);
         p.x = p.x + pinv.x; p.y = p.y + pinv.y; p.z = p.z + pinv.z;
         rxyz[j] = p.x; rxyz[nr + j] = p.y; rxyz[2 * nr + j] = p.z;
-        const float4 pp = pot_bead(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w, nullptr);
+        // the lookup k_modes' phase A makes, so its per-bead values are under test
+        const float4 pp = pot_bead_flat(lInner, lOuter, p.x, p.y, p.z, rec.mtype[j], x.w);
         const V3 f = matvec(R, pp.x, pp.y, pp.z);
         rpot[j] = f.x; rpot[nr + j] = f.y; rpot[2 * nr + j] = f.z;
     }
